@@ -1,0 +1,165 @@
+/*
+ * vonkarman_hip.h -- C ABI of the MI355X (gfx950) backend for vonkarman's
+ * periodic pseudospectral time step.
+ *
+ * Two objects replace the reference's Rust traits on this path:
+ *   vk_fft_plan  <-> trait FftBackend<f64>   (vonkarman-fft/src/backend.rs:9-25)
+ *   vk_solver    <-> trait Domain<f64> as implemented by Periodic3D
+ *                    (vonkarman-core/src/domain.rs:39-81,
+ *                     vonkarman-periodic/src/solver.rs:12-567)
+ *
+ * Conventions (those of the reference's ndarray buffers):
+ *   real field      : double[nx][ny][nz], C order
+ *   spectral field  : interleaved complex double[nx][ny][nz/2+1][2]
+ *   vector field    : three fields back to back (x, y, z components)
+ * Host pointers are caller-owned; the library owns all device memory.  Every
+ * call returns VK_OK (0) or an error code, and is complete when it returns
+ * (one HIP stream per handle).  A handle is not thread-safe (the reference
+ * objects are Send, not Sync).  vk_last_error() describes the last failure
+ * of the calling thread.
+ */
+#ifndef VONKARMAN_HIP_H
+#define VONKARMAN_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VK_OK 0
+#define VK_ERR_ARG 1         /* invalid argument (null handle, bad size) */
+#define VK_ERR_UNSUPPORTED 2 /* length or precision not built */
+#define VK_ERR_HIP 3         /* HIP runtime failure (message in vk_last_error) */
+#define VK_ERR_NOMEM 4       /* device allocation failed */
+
+#define VK_PREC_F64 0   /* all arithmetic in FP64 */
+#define VK_PREC_MIXED 1 /* FFTs, u x w and projection in FP32; state, phi
+                           table, twiddle generation and reductions in FP64 */
+
+/* Thread-local description of the last error ("" if none). */
+const char *vk_last_error(void);
+/* Library version, major*10000 + minor*100 + patch. */
+int vk_version(void);
+/* Number of visible HIP devices (0 when no GPU). */
+int vk_device_count(int *count);
+
+/* ------------------------------------------------------------------------ */
+/* FftBackend<f64>                                                          */
+/* ------------------------------------------------------------------------ */
+typedef struct vk_fft_plan vk_fft_plan;
+
+/* CufftBackend::new (vonkarman-fft/src/cufft.rs:181-234): plans for an
+ * nx*ny*nz grid.  Each length must be even and of the form 2^a 3^b in
+ * [8, 768]. */
+int vk_fft_plan_create(int nx, int ny, int nz, vk_fft_plan **out);
+/* Drop for CufftBackend (cufft.rs:237-246). */
+int vk_fft_plan_destroy(vk_fft_plan *plan);
+/* FftBackend::r2c_3d (backend.rs:10-13): unnormalised forward transform,
+ * R2C along z then C2C along y and x.  in: real[nx][ny][nz],
+ * out: complex[nx][ny][nz/2+1] (host pointers). */
+int vk_fft_r2c_3d(vk_fft_plan *plan, const double *in, double *out);
+/* FftBackend::c2r_3d (backend.rs:15-18): inverse transform with the
+ * 1/(nx*ny*nz) normalisation; the imaginary parts of the kz=0 and kz=nz/2
+ * inputs are ignored.  Host pointers. */
+int vk_fft_c2r_3d(vk_fft_plan *plan, const double *in, double *out);
+/* The same two transforms on device pointers (same layouts). */
+int vk_fft_r2c_3d_device(vk_fft_plan *plan, const double *d_in, double *d_out);
+int vk_fft_c2r_3d_device(vk_fft_plan *plan, const double *d_in, double *d_out);
+/* FftBackend::name / precision_digits (backend.rs:21-24). */
+const char *vk_fft_name(const vk_fft_plan *plan);
+int vk_fft_precision_digits(const vk_fft_plan *plan);
+/* Device time of the last transform on this plan, in milliseconds, and the
+ * kernel part of it (excluding host transfers). */
+int vk_fft_last_time_ms(const vk_fft_plan *plan, double *kernel_ms);
+
+/* ------------------------------------------------------------------------ */
+/* Domain<f64> / Periodic3D                                                 */
+/* ------------------------------------------------------------------------ */
+typedef struct vk_solver vk_solver;
+
+/* ScalarDiagnostics (vonkarman-diag/src/scalar.rs:6-18): the 11 columns of
+ * the reference's diagnostics.parquet (diagnostics_writer.rs:25-37). */
+typedef struct vk_diag {
+    uint64_t step;
+    double time;
+    double dt;
+    double energy;
+    double enstrophy;
+    double helicity;
+    double superhelicity;
+    double max_vorticity;
+    double energy_dissipation_rate;
+    double helicity_dissipation_rate;
+    double cfl_number;
+} vk_diag;
+
+/* Per-pass device time of vk_solver_step_timed, milliseconds summed over the
+ * timed steps (the NL passes are summed over the 4 stages of each step). */
+typedef struct vk_timing {
+    double x_inverse;   /* x pass: pad, i*kx terms, inverse DFT    */
+    double y_inverse;   /* y pass: pad, curl, inverse DFT           */
+    double z_cross;     /* z pass: C2R, u x w, R2C                  */
+    double y_forward;   /* y pass: forward DFT, truncate            */
+    double x_forward;   /* x pass: forward DFT, truncate, Leray, stage */
+    double cfl;         /* CFL max|u| passes, dt update, phi table  */
+    double total;       /* whole timed region                       */
+    int steps;
+} vk_timing;
+
+/* Periodic3D::new without the IC (solver.rs:38-108): an n^3 grid on a box of
+ * side l, viscosity nu.  n: power of two in [8, 512]. precision: VK_PREC_*.
+ * The state starts at zero, dt at 0, time 0, step 0. */
+int vk_solver_create(int n, double l, double nu, int precision, vk_solver **out);
+int vk_solver_destroy(vk_solver *s);
+/* u_hat (the reference's Periodic3D.u_hat, solver.rs:14) in and out, as
+ * interleaved complex[3][n][n][n/2+1] host arrays. */
+int vk_solver_set_state(vk_solver *s, const double *uhat);
+int vk_solver_get_state(vk_solver *s, double *uhat);
+/* Same layout, device pointers. */
+int vk_solver_set_state_device(vk_solver *s, const double *d_uhat);
+int vk_solver_get_state_device(vk_solver *s, double *d_uhat);
+/* Periodic3D::new tail (solver.rs:88-93): dt from the CFL rule and the ETD
+ * coefficient table.  Adaptive mode. */
+int vk_solver_init_dt(vk_solver *s);
+/* New entry point (the reference has no fixed-dt mode): set dt and rebuild
+ * the table; fixed != 0 skips the CFL update of step(). */
+int vk_solver_set_dt(vk_solver *s, double dt, int fixed);
+/* Periodic3D::from_checkpoint clock restore (solver.rs:127-159). */
+int vk_solver_set_clock(vk_solver *s, double time, uint64_t step);
+/* Domain::step (solver.rs:383-400), nsteps times: CFL dt update (adaptive
+ * mode), ETD-RK4 step (solver.rs:221-379), time += dt, step += 1. */
+int vk_solver_step(vk_solver *s, int nsteps);
+/* vk_solver_step with per-pass device timing accumulated into *t. */
+int vk_solver_step_timed(vk_solver *s, int nsteps, vk_timing *t);
+/* Domain getters (domain.rs:43-68). */
+int vk_solver_time(vk_solver *s, double *time);
+int vk_solver_dt(vk_solver *s, double *dt);
+int vk_solver_step_count(vk_solver *s, uint64_t *step);
+int vk_solver_energy(vk_solver *s, double *v);        /* solver.rs:412-434 */
+int vk_solver_enstrophy(vk_solver *s, double *v);     /* solver.rs:436-465 */
+int vk_solver_helicity(vk_solver *s, double *v);      /* solver.rs:467-497 */
+int vk_solver_superhelicity(vk_solver *s, double *v); /* solver.rs:499-535 */
+int vk_solver_max_vorticity(vk_solver *s, double *v); /* solver.rs:537-567 */
+int vk_solver_cfl_dt(vk_solver *s, double *v);        /* solver.rs:569-577 */
+/* ScalarDiagnostics::from_domain (scalar.rs:22-46) in one device pass. */
+int vk_solver_diagnostics(vk_solver *s, vk_diag *d);
+/* energy_spectrum (vonkarman-diag/src/spectrum.rs:13-60): writes
+ * min(cap, n_shells) shells and stores n_shells = ceil(sqrt(3) n/2) + 1. */
+int vk_solver_energy_spectrum(vk_solver *s, double *out, int cap, int *n_shells);
+/* compute_nonlinear (vonkarman-periodic/src/nonlinear.rs:15-39) on a host
+ * state: nhat = P trunc FFT(u x curl u).  Does not touch the solver state. */
+int vk_solver_nonlinear(vk_solver *s, const double *uhat, double *nhat);
+/* Domain::snapshot physical fields (solver.rs:589-623): velocity and
+ * vorticity as real[3][n][n][n]; either pointer may be NULL. */
+int vk_solver_physical(vk_solver *s, double *u_phys, double *w_phys);
+/* Device memory held by the solver, in bytes. */
+int vk_solver_device_bytes(const vk_solver *s, uint64_t *bytes);
+
+/* Host-side Taylor-Green initial velocity (vonkarman-periodic/src/ic/
+ * taylor_green.rs:10-38): real[3][n][n][n]. */
+int vk_ic_taylor_green(int n, double l, double *u_phys);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

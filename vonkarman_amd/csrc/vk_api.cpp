@@ -1,0 +1,776 @@
+// vk_api.cpp -- the C ABI (include/vonkarman_hip.h): plan and solver objects,
+// device memory ownership, and the launch sequence of one ETD-RK4 step.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/vonkarman_hip.h"
+#include "vk_fft.hpp"
+#include "vk_launch.h"
+
+using namespace vk;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define VK_HIP(call)                                                                                  \
+    do {                                                                                              \
+        hipError_t e_ = (call);                                                                       \
+        if (e_ != hipSuccess)                                                                         \
+            return fail(VK_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));               \
+    } while (0)
+
+#define VK_TRY(call)                                                                   \
+    do {                                                                               \
+        int r_ = (call);                                                               \
+        if (r_ != 0) {                                                                 \
+            if (r_ == 3) {                                                             \
+                hipError_t e_ = hipGetLastError();                                     \
+                return fail(VK_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+            }                                                                          \
+            return fail(r_ == 2 ? VK_ERR_UNSUPPORTED : r_, std::string(#call) + " failed"); \
+        }                                                                              \
+    } while (0)
+
+bool length_ok(int n) {
+    static const int lens[] = {8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512, 768};
+    for (int l : lens)
+        if (l == n) return true;
+    return false;
+}
+
+int round_up(int a, int b) { return (a + b - 1) / b * b; }
+
+// W_L[q] = exp(-2 pi i q / L), computed in extended precision with the angle
+// reduced to the first octant, then rounded once.
+std::vector<double> twiddles(int L) {
+    std::vector<double> w(2 * L);
+    const long double pi = 3.141592653589793238462643383279502884L;
+    for (int q = 0; q < L; ++q) {
+        // angle = 2 pi q / L = (pi/4) * (8q/L); use symmetry for accuracy
+        const long double a = 2.0L * pi * (long double)q / (long double)L;
+        w[2 * q] = (double)cosl(a);
+        w[2 * q + 1] = (double)-sinl(a);
+    }
+    // exact values at the quarter points
+    for (int q = 0; q < L; ++q) {
+        if ((4LL * q) % L == 0) {
+            const int k = (int)((4LL * q) / L);
+            const double c[4] = {1, 0, -1, 0}, s[4] = {0, -1, 0, 1};
+            w[2 * q] = c[k];
+            w[2 * q + 1] = s[k];
+        }
+    }
+    return w;
+}
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+int dev_alloc(DevBuf &b, size_t bytes, size_t &total) {
+    if (bytes == 0) return 0;
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) return fail(VK_ERR_NOMEM, "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    b.bytes = bytes;
+    total += bytes;
+    return 0;
+}
+
+void dev_free(DevBuf &b) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+}
+
+int upload_twiddles(DevBuf &b, int L, Prec p, size_t &total) {
+    std::vector<double> w = twiddles(L);
+    if (p == F64) {
+        if (int r = dev_alloc(b, w.size() * sizeof(double), total)) return r;
+        VK_HIP(hipMemcpy(b.p, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> f(w.begin(), w.end());
+        if (int r = dev_alloc(b, f.size() * sizeof(float), total)) return r;
+        VK_HIP(hipMemcpy(b.p, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+Strided make_strided(int ncols, int hp, int hvalid, int in_inner, long long in_os, long long in_ls, long long in_fs,
+                     int out_inner, long long out_os, long long out_ls, long long out_fs) {
+    Strided g;
+    g.ncols = ncols;
+    g.hp = hp;
+    g.hvalid = hvalid;
+    g.in_inner = in_inner;
+    g.in_os = in_os;
+    g.in_ls = in_ls;
+    g.in_fs = in_fs;
+    g.out_inner = out_inner;
+    g.out_os = out_os;
+    g.out_ls = out_ls;
+    g.out_fs = out_fs;
+    return g;
+}
+
+}  // namespace
+
+// ===========================================================================
+// FftBackend<f64>
+// ===========================================================================
+struct vk_fft_plan {
+    int nx, ny, nz, hz, hp;
+    hipStream_t st = nullptr;
+    DevBuf tw_x, tw_y, tw_z;
+    DevBuf real, spec, t1, t2;
+    size_t total = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
+    double last_ms = 0.0, last_kernel_ms = 0.0;
+    std::string name;
+};
+
+static int plan_c2r_kernels(vk_fft_plan *p, const void *spec_padded) {
+    // x inverse: t1 natural [ix][iy][hp] -> t2 [iy][ix][hp]
+    const int nx = p->nx, ny = p->ny, nz = p->nz, hp = p->hp, hz = p->hz;
+    Strided gx = make_strided(ny * hp, hp, hz, ny * hp, 0, (long long)ny * hp, 0, hp, (long long)nx * hp, hp, 0);
+    VK_TRY(launch_inv(INV_PLAIN, nx, nx, F64, F64, spec_padded, p->t2.p, p->tw_x.p, gx, nx, 1.0, 1.0, p->st));
+    // y inverse: t2 [iy][ix][hp] -> t1 [ix][iy][hp]
+    Strided gy = make_strided(nx * hp, hp, hz, nx * hp, 0, (long long)nx * hp, 0, hp, (long long)ny * hp, hp, 0);
+    VK_TRY(launch_inv(INV_PLAIN, ny, ny, F64, F64, p->t2.p, p->t1.p, p->tw_y.p, gy, ny, 1.0, 1.0, p->st));
+    // z C2R with the 1/(nx ny nz) normalisation
+    const double scale = 1.0 / ((double)nx * (double)ny * (double)nz);
+    VK_TRY(launch_z_c2r(nz, F64, p->t1.p, static_cast<double *>(p->real.p), (long long)nx * ny, hp, scale, p->tw_z.p,
+                        p->st));
+    return 0;
+}
+
+static int plan_r2c_kernels(vk_fft_plan *p) {
+    const int nx = p->nx, ny = p->ny, nz = p->nz, hp = p->hp, hz = p->hz;
+    // z R2C: real -> t1 [ix][iy][hp]
+    VK_TRY(launch_z_r2c(nz, F64, static_cast<const double *>(p->real.p), p->t1.p, (long long)nx * ny, hp, p->tw_z.p,
+                        p->st));
+    // y forward: t1 [ix][iy][hp] -> t2 [iy][ix][hp]
+    Strided gy = make_strided(nx * hp, hp, hz, hp, (long long)ny * hp, hp, 0, nx * hp, 0, (long long)nx * hp, 0);
+    VK_TRY(launch_fwd(FWD_PLAIN, ny, ny, 1, F64, p->t1.p, p->t2.p, p->tw_y.p, gy, ny, 1.0, 1.0, nullptr, p->st));
+    // x forward: t2 [iy][ix][hp] -> t1 natural [ix][iy][hp]
+    Strided gx = make_strided(ny * hp, hp, hz, hp, (long long)nx * hp, hp, 0, ny * hp, 0, (long long)ny * hp, 0);
+    VK_TRY(launch_fwd(FWD_PLAIN, nx, nx, 1, F64, p->t2.p, p->t1.p, p->tw_x.p, gx, nx, 1.0, 1.0, nullptr, p->st));
+    return 0;
+}
+
+extern "C" {
+
+const char *vk_last_error(void) { return g_err.c_str(); }
+int vk_version(void) { return 100; }
+
+int vk_device_count(int *count) {
+    if (!count) return fail(VK_ERR_ARG, "null pointer");
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    *count = c;
+    return VK_OK;
+}
+
+int vk_fft_plan_create(int nx, int ny, int nz, vk_fft_plan **out) {
+    if (!out) return fail(VK_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    if (!length_ok(nx) || !length_ok(ny) || !length_ok(nz))
+        return fail(VK_ERR_UNSUPPORTED, "vk_fft_plan_create: lengths must be 2^a 3^b in [8, 768]");
+    auto *p = new vk_fft_plan();
+    p->nx = nx;
+    p->ny = ny;
+    p->nz = nz;
+    p->hz = nz / 2 + 1;
+    p->hp = round_up(p->hz, 8);
+    p->name = "vonkarman-hip (gfx950 Stockham, fp64)";
+    int r = 0;
+    const size_t ncplx = (size_t)nx * ny * p->hp;
+    if ((r = upload_twiddles(p->tw_x, nx, F64, p->total)) || (r = upload_twiddles(p->tw_y, ny, F64, p->total)) ||
+        (r = upload_twiddles(p->tw_z, nz, F64, p->total)) ||
+        (r = dev_alloc(p->real, (size_t)nx * ny * nz * sizeof(double), p->total)) ||
+        (r = dev_alloc(p->spec, (size_t)nx * ny * p->hz * 16, p->total)) ||
+        (r = dev_alloc(p->t1, ncplx * 16, p->total)) || (r = dev_alloc(p->t2, ncplx * 16, p->total))) {
+        std::string msg = g_err;
+        vk_fft_plan_destroy(p);
+        g_err = msg;
+        return r;
+    }
+    if (hipStreamCreateWithFlags(&p->st, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&p->ev0) != hipSuccess || hipEventCreate(&p->ev1) != hipSuccess ||
+        hipEventCreate(&p->ev2) != hipSuccess || hipEventCreate(&p->ev3) != hipSuccess) {
+        vk_fft_plan_destroy(p);
+        return fail(VK_ERR_HIP, "vk_fft_plan_create: stream/event creation failed");
+    }
+    *out = p;
+    return VK_OK;
+}
+
+int vk_fft_plan_destroy(vk_fft_plan *p) {
+    if (!p) return VK_OK;
+    if (p->st) (void)hipStreamSynchronize(p->st);
+    dev_free(p->tw_x);
+    dev_free(p->tw_y);
+    dev_free(p->tw_z);
+    dev_free(p->real);
+    dev_free(p->spec);
+    dev_free(p->t1);
+    dev_free(p->t2);
+    for (hipEvent_t e : {p->ev0, p->ev1, p->ev2, p->ev3})
+        if (e) (void)hipEventDestroy(e);
+    if (p->st) (void)hipStreamDestroy(p->st);
+    delete p;
+    return VK_OK;
+}
+
+static int fft_finish(vk_fft_plan *p) {
+    VK_HIP(hipStreamSynchronize(p->st));
+    float a = 0, b = 0;
+    VK_HIP(hipEventElapsedTime(&a, p->ev0, p->ev3));
+    VK_HIP(hipEventElapsedTime(&b, p->ev1, p->ev2));
+    p->last_ms = a;
+    p->last_kernel_ms = b;
+    return VK_OK;
+}
+
+static int fft_c2r_impl(vk_fft_plan *p, const double *in, double *out, bool device) {
+    if (!p || !in || !out) return fail(VK_ERR_ARG, "vk_fft_c2r_3d: null argument");
+    const size_t spec_bytes = (size_t)p->nx * p->ny * p->hz * 16;
+    const size_t real_bytes = (size_t)p->nx * p->ny * p->nz * sizeof(double);
+    VK_HIP(hipEventRecord(p->ev0, p->st));
+    VK_HIP(hipMemcpyAsync(p->spec.p, in, spec_bytes, device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, p->st));
+    VK_HIP(hipEventRecord(p->ev1, p->st));
+    VK_TRY(launch_repack(p->spec.p, p->t1.p, (long long)p->nx * p->ny, p->hz, p->hp, p->hz, 1, p->st));
+    if (int r = plan_c2r_kernels(p, p->t1.p)) return r;
+    VK_HIP(hipEventRecord(p->ev2, p->st));
+    VK_HIP(hipMemcpyAsync(out, p->real.p, real_bytes, device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, p->st));
+    VK_HIP(hipEventRecord(p->ev3, p->st));
+    return fft_finish(p);
+}
+
+static int fft_r2c_impl(vk_fft_plan *p, const double *in, double *out, bool device) {
+    if (!p || !in || !out) return fail(VK_ERR_ARG, "vk_fft_r2c_3d: null argument");
+    const size_t spec_bytes = (size_t)p->nx * p->ny * p->hz * 16;
+    const size_t real_bytes = (size_t)p->nx * p->ny * p->nz * sizeof(double);
+    VK_HIP(hipEventRecord(p->ev0, p->st));
+    VK_HIP(hipMemcpyAsync(p->real.p, in, real_bytes, device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, p->st));
+    VK_HIP(hipEventRecord(p->ev1, p->st));
+    if (int r = plan_r2c_kernels(p)) return r;
+    VK_TRY(launch_repack(p->t1.p, p->spec.p, (long long)p->nx * p->ny, p->hp, p->hz, p->hz, 1, p->st));
+    VK_HIP(hipEventRecord(p->ev2, p->st));
+    VK_HIP(hipMemcpyAsync(out, p->spec.p, spec_bytes, device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, p->st));
+    VK_HIP(hipEventRecord(p->ev3, p->st));
+    return fft_finish(p);
+}
+
+int vk_fft_r2c_3d(vk_fft_plan *p, const double *in, double *out) { return fft_r2c_impl(p, in, out, false); }
+int vk_fft_c2r_3d(vk_fft_plan *p, const double *in, double *out) { return fft_c2r_impl(p, in, out, false); }
+int vk_fft_r2c_3d_device(vk_fft_plan *p, const double *in, double *out) { return fft_r2c_impl(p, in, out, true); }
+int vk_fft_c2r_3d_device(vk_fft_plan *p, const double *in, double *out) { return fft_c2r_impl(p, in, out, true); }
+const char *vk_fft_name(const vk_fft_plan *p) { return p ? p->name.c_str() : ""; }
+int vk_fft_precision_digits(const vk_fft_plan *p) { return p ? 15 : 0; }
+int vk_fft_last_time_ms(const vk_fft_plan *p, double *kernel_ms) {
+    if (!p || !kernel_ms) return fail(VK_ERR_ARG, "null argument");
+    *kernel_ms = p->last_kernel_ms;
+    return VK_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Domain<f64> / Periodic3D
+// ===========================================================================
+struct vk_solver {
+    int n, m, h, hp;
+    double l, ks, nu, dx;
+    Prec wp;
+    bool fixed = false;
+    hipStream_t st = nullptr;
+    long long S;  // complex elements per state field (n*n*hp)
+    DevBuf u, x, n1, sacc;  // state-precision (FP64) arrays [3][S]
+    DevBuf A, B;            // work-precision intermediates
+    DevBuf tab;             // ETD table, 7 x (qmax+1)
+    int qmax;
+    DevBuf sc;              // Scalars
+    DevBuf tw_n, tw_m;      // work-precision twiddles
+    DevBuf partials, spec;  // diagnostics
+    int nblocks_diag, nshells;
+    DevBuf staging;         // unpadded [3][n][n][h] complex
+    size_t total = 0;
+    hipEvent_t ev[8] = {};
+    Scalars *dsc() { return static_cast<Scalars *>(sc.p); }
+    size_t wbytes() const { return wp == F64 ? 16 : 8; }
+};
+
+namespace {
+
+// one nonlinear evaluation N = P trunc FFT(u x curl u) of the state-layout
+// field `src`, fused with the ETD-RK4 stage epilogue `sa`.
+int nl_eval(vk_solver *s, const void *src, StageArgs &sa, vk_timing *t, hipEvent_t *ev) {
+    const int N = s->n, M = s->m, Hp = s->hp, H = s->h;
+    const long long NMH = (long long)N * M * Hp, MMH = (long long)M * M * Hp, NH = (long long)N * Hp;
+    const long long MH = (long long)M * Hp;
+    auto mark = [&](int i) -> int {
+        if (t) VK_HIP(hipEventRecord(ev[i], s->st));
+        return 0;
+    };
+    if (int r = mark(0)) return r;
+    // P1: x inverse with padding: u[3][N][N][Hp] -> A[5][N][M][Hp] (y-major)
+    Strided g1 = make_strided((int)NH, Hp, H, (int)NH, 0, NH, s->S, Hp, MH, Hp, NMH);
+    VK_TRY(launch_inv(INV_U3DX, M, N, F64, s->wp, src, s->A.p, s->tw_m.p, g1, N, s->ks, 1.0, s->st));
+    if (int r = mark(1)) return r;
+    // P2: y inverse with padding, curl completed: A -> B[6][M][M][Hp]
+    Strided g2 = make_strided((int)MH, Hp, H, (int)MH, 0, MH, NMH, Hp, MH, Hp, MMH);
+    VK_TRY(launch_inv(INV_UOMEGA, M, N, s->wp, s->wp, s->A.p, s->B.p, s->tw_m.p, g2, N, s->ks, 1.0, s->st));
+    if (int r = mark(2)) return r;
+    // P3: z C2R, u x w, R2C, truncate: B -> C (in A) [3][M][M][Hp]
+    const double nt = (double)N * N * N;
+    const double inv_scale = 1.0 / ((double)M * M * M / nt);
+    VK_TRY(launch_z_nl(M, s->wp, s->B.p, s->A.p, (long long)M * M, Hp, MMH, MMH, 1.0 / nt, inv_scale, s->tw_m.p, s->st));
+    if (int r = mark(3)) return r;
+    // P2': y forward, truncate: C (A) -> D (B) [3][N][M][Hp] (y-major)
+    Strided g3 = make_strided((int)MH, Hp, H, Hp, MH, Hp, MMH, (int)MH, 0, MH, NMH);
+    VK_TRY(launch_fwd(FWD_PLAIN, M, N, 3, s->wp, s->A.p, s->B.p, s->tw_m.p, g3, N, s->ks, 1.0, nullptr, s->st));
+    if (int r = mark(4)) return r;
+    // P1': x forward, truncate, Leray, stage epilogue: D (B) -> state arrays
+    Strided g4 = make_strided((int)NH, Hp, H, Hp, MH, Hp, NMH, (int)NH, 0, NH, s->S);
+    VK_TRY(launch_fwd(FWD_STAGE, M, N, 3, s->wp, s->B.p, nullptr, s->tw_m.p, g4, N, s->ks, 1.0, &sa, s->st));
+    if (int r = mark(5)) return r;
+    return 0;
+}
+
+// max |v|^2 over the unpadded physical grid of u (omega=false) or curl u.
+int max_pass(vk_solver *s, bool omega) {
+    const int N = s->n, Hp = s->hp, H = s->h;
+    const long long NH = (long long)N * Hp, NNH = (long long)N * N * Hp;
+    VK_TRY(launch_fill_u64(&s->dsc()->umax2_bits, 0ull, 1, s->st));
+    Strided g1 = make_strided((int)NH, Hp, H, (int)NH, 0, NH, s->S, Hp, NH, Hp, NNH);
+    VK_TRY(launch_inv(omega ? INV_U3DX : INV_U3, N, N, F64, s->wp, s->u.p, s->A.p, s->tw_n.p, g1, N, s->ks, 1.0,
+                      s->st));
+    Strided g2 = make_strided((int)NH, Hp, H, (int)NH, 0, NH, NNH, Hp, NH, Hp, NNH);
+    VK_TRY(launch_inv(omega ? INV_OMEGA : INV_U3, N, N, s->wp, s->wp, s->A.p, s->B.p, s->tw_n.p, g2, N, s->ks, 1.0,
+                      s->st));
+    const double nt = (double)N * N * N;
+    VK_TRY(launch_z_max(N, s->wp, s->B.p, (long long)N * N, Hp, NNH, 1.0 / nt, s->tw_n.p, s->dsc(), s->st));
+    return 0;
+}
+
+StageArgs stage_args(vk_solver *s, int stage, void *out) {
+    StageArgs a{};
+    a.stage = stage;
+    a.out = out;
+    a.u = s->u.p;
+    a.n1 = s->n1.p;
+    a.sacc = s->sacc.p;
+    a.fs = s->S;
+    a.tab = static_cast<const double *>(s->tab.p);
+    a.tab_stride = s->qmax + 1;
+    a.sc = s->dsc();
+    return a;
+}
+
+int read_scalars(vk_solver *s, Scalars &h) {
+    VK_HIP(hipMemcpyAsync(&h, s->sc.p, sizeof(Scalars), hipMemcpyDeviceToHost, s->st));
+    VK_HIP(hipStreamSynchronize(s->st));
+    return 0;
+}
+
+double bits_to_double(unsigned long long b) {
+    double d;
+    std::memcpy(&d, &b, sizeof d);
+    return d;
+}
+
+int check(vk_solver *s) {
+    if (!s) return fail(VK_ERR_ARG, "null solver handle");
+    return 0;
+}
+
+int one_step(vk_solver *s, vk_timing *t) {
+    float ms = 0;
+    if (!s->fixed) {
+        if (t) VK_HIP(hipEventRecord(s->ev[6], s->st));
+        if (int r = max_pass(s, false)) return r;
+        VK_TRY(launch_cfl_update(s->dsc(), s->dx, s->nu, 0.5, 0, s->st));
+        VK_TRY(launch_etd_table(static_cast<double *>(s->tab.p), s->qmax, s->nu, s->ks, s->dsc(), 0, s->st));
+        if (t) {
+            VK_HIP(hipEventRecord(s->ev[7], s->st));
+            VK_HIP(hipEventSynchronize(s->ev[7]));
+            VK_HIP(hipEventElapsedTime(&ms, s->ev[6], s->ev[7]));
+            t->cfl += ms;
+        }
+    }
+    static const int stages[4] = {ST_1, ST_2, ST_3, ST_4};
+    for (int k = 0; k < 4; ++k) {
+        const void *src = k == 0 ? s->u.p : s->x.p;
+        void *dst = k == 3 ? s->u.p : s->x.p;
+        StageArgs sa = stage_args(s, stages[k], dst);
+        if (int r = nl_eval(s, src, sa, t, s->ev)) return r;
+        if (t) {
+            VK_HIP(hipEventSynchronize(s->ev[5]));
+            double *acc[5] = {&t->x_inverse, &t->y_inverse, &t->z_cross, &t->y_forward, &t->x_forward};
+            for (int i = 0; i < 5; ++i) {
+                VK_HIP(hipEventElapsedTime(&ms, s->ev[i], s->ev[i + 1]));
+                *acc[i] += ms;
+            }
+        }
+    }
+    VK_TRY(launch_clock_advance(s->dsc(), s->st));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vk_solver_create(int n, double l, double nu, int precision, vk_solver **out) {
+    if (!out) return fail(VK_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    if (n < 8 || n > 512 || (n & (n - 1)))
+        return fail(VK_ERR_UNSUPPORTED, "vk_solver_create: n must be a power of two in [8, 512]");
+    if (precision != VK_PREC_F64 && precision != VK_PREC_MIXED)
+        return fail(VK_ERR_UNSUPPORTED, "vk_solver_create: unknown precision");
+    if (!(l > 0.0)) return fail(VK_ERR_ARG, "vk_solver_create: box length must be positive");
+    auto *s = new vk_solver();
+    s->n = n;
+    s->m = 3 * n / 2;  // GridSpec::padded_3half (vonkarman-core/src/field.rs:53-62)
+    s->h = n / 2 + 1;
+    s->wp = precision == VK_PREC_F64 ? F64 : F32;
+    const int row_elems = s->wp == F64 ? 8 : 16;  // one 128-byte row per tile position
+    s->hp = round_up(s->h, row_elems);
+    s->l = l;
+    s->ks = 2.0 * 3.14159265358979323846 / l;  // spectral_ops.rs:52
+    s->nu = nu;
+    s->dx = l / (double)n;  // GridSpec::dx (field.rs:30-32)
+    s->S = (long long)n * n * s->hp;
+    s->qmax = 3 * (n / 2) * (n / 2);
+    const long long N = n, M = s->m, Hp = s->hp;
+    const size_t a_elems = (size_t)std::max(5 * N * M * Hp, 3 * M * M * Hp);
+    const size_t b_elems = (size_t)std::max(6 * M * M * Hp, 3 * N * M * Hp);
+    const double kmax = std::sqrt(3.0 * (double)(n / 2) * (double)(n / 2));
+    s->nshells = (int)std::ceil(kmax) + 1;  // spectrum.rs:26-30
+    s->nblocks_diag = 1024;
+    int r = 0;
+    const size_t state_bytes = (size_t)s->S * 3 * 16;
+    if ((r = dev_alloc(s->u, state_bytes, s->total)) || (r = dev_alloc(s->x, state_bytes, s->total)) ||
+        (r = dev_alloc(s->n1, state_bytes, s->total)) || (r = dev_alloc(s->sacc, state_bytes, s->total)) ||
+        (r = dev_alloc(s->A, a_elems * s->wbytes(), s->total)) ||
+        (r = dev_alloc(s->B, b_elems * s->wbytes(), s->total)) ||
+        (r = dev_alloc(s->tab, (size_t)7 * (s->qmax + 1) * sizeof(double), s->total)) ||
+        (r = dev_alloc(s->sc, sizeof(Scalars), s->total)) ||
+        (r = upload_twiddles(s->tw_n, n, s->wp, s->total)) ||
+        (r = upload_twiddles(s->tw_m, s->m, s->wp, s->total)) ||
+        (r = dev_alloc(s->partials, (size_t)s->nblocks_diag * (4 + s->nshells) * sizeof(double), s->total)) ||
+        (r = dev_alloc(s->spec, (size_t)s->nshells * sizeof(double), s->total)) ||
+        (r = dev_alloc(s->staging, (size_t)3 * n * n * s->h * 16, s->total))) {
+        std::string msg = g_err;
+        vk_solver_destroy(s);
+        g_err = msg;
+        return r;
+    }
+    if (hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess) {
+        vk_solver_destroy(s);
+        return fail(VK_ERR_HIP, "vk_solver_create: stream creation failed");
+    }
+    for (auto &e : s->ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            vk_solver_destroy(s);
+            return fail(VK_ERR_HIP, "vk_solver_create: event creation failed");
+        }
+    Scalars h0{};
+    if (hipMemcpy(s->sc.p, &h0, sizeof(Scalars), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(s->u.p, 0, state_bytes) != hipSuccess) {
+        vk_solver_destroy(s);
+        return fail(VK_ERR_HIP, "vk_solver_create: initialisation failed");
+    }
+    *out = s;
+    return VK_OK;
+}
+
+int vk_solver_destroy(vk_solver *s) {
+    if (!s) return VK_OK;
+    if (s->st) (void)hipStreamSynchronize(s->st);
+    for (DevBuf *b : {&s->u, &s->x, &s->n1, &s->sacc, &s->A, &s->B, &s->tab, &s->sc, &s->tw_n, &s->tw_m,
+                      &s->partials, &s->spec, &s->staging})
+        dev_free(*b);
+    for (auto &e : s->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (s->st) (void)hipStreamDestroy(s->st);
+    delete s;
+    return VK_OK;
+}
+
+static int set_state_impl(vk_solver *s, const double *uhat, bool device) {
+    if (int r = check(s)) return r;
+    if (!uhat) return fail(VK_ERR_ARG, "null state pointer");
+    const size_t bytes = (size_t)3 * s->n * s->n * s->h * 16;
+    VK_HIP(hipMemcpyAsync(s->staging.p, uhat, bytes, device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s->st));
+    VK_TRY(launch_repack(s->staging.p, s->u.p, (long long)3 * s->n * s->n, s->h, s->hp, s->h, 1, s->st));
+    VK_HIP(hipStreamSynchronize(s->st));
+    return VK_OK;
+}
+static int get_state_impl(vk_solver *s, double *uhat, bool device, const void *src = nullptr) {
+    if (int r = check(s)) return r;
+    if (!uhat) return fail(VK_ERR_ARG, "null state pointer");
+    const size_t bytes = (size_t)3 * s->n * s->n * s->h * 16;
+    VK_TRY(launch_repack(src ? src : s->u.p, s->staging.p, (long long)3 * s->n * s->n, s->hp, s->h, s->h, 1, s->st));
+    VK_HIP(hipMemcpyAsync(uhat, s->staging.p, bytes, device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s->st));
+    VK_HIP(hipStreamSynchronize(s->st));
+    return VK_OK;
+}
+int vk_solver_set_state(vk_solver *s, const double *u) { return set_state_impl(s, u, false); }
+int vk_solver_get_state(vk_solver *s, double *u) { return get_state_impl(s, u, false); }
+int vk_solver_set_state_device(vk_solver *s, const double *u) { return set_state_impl(s, u, true); }
+int vk_solver_get_state_device(vk_solver *s, double *u) { return get_state_impl(s, u, true); }
+
+int vk_solver_init_dt(vk_solver *s) {
+    if (int r = check(s)) return r;
+    if (int r = max_pass(s, false)) return r;
+    VK_TRY(launch_cfl_update(s->dsc(), s->dx, s->nu, 0.5, 1, s->st));
+    VK_TRY(launch_etd_table(static_cast<double *>(s->tab.p), s->qmax, s->nu, s->ks, s->dsc(), 1, s->st));
+    VK_HIP(hipStreamSynchronize(s->st));
+    s->fixed = false;
+    return VK_OK;
+}
+
+int vk_solver_set_dt(vk_solver *s, double dt, int fixed) {
+    if (int r = check(s)) return r;
+    if (!(dt > 0.0)) return fail(VK_ERR_ARG, "vk_solver_set_dt: dt must be positive");
+    VK_HIP(hipMemcpyAsync(&s->dsc()->dt, &dt, sizeof(double), hipMemcpyHostToDevice, s->st));
+    VK_TRY(launch_etd_table(static_cast<double *>(s->tab.p), s->qmax, s->nu, s->ks, s->dsc(), 1, s->st));
+    VK_HIP(hipStreamSynchronize(s->st));
+    s->fixed = fixed != 0;
+    return VK_OK;
+}
+
+int vk_solver_set_clock(vk_solver *s, double time, uint64_t step) {
+    if (int r = check(s)) return r;
+    Scalars h;
+    if (int r = read_scalars(s, h)) return r;
+    h.time = time;
+    h.step = step;
+    VK_HIP(hipMemcpyAsync(s->sc.p, &h, sizeof(Scalars), hipMemcpyHostToDevice, s->st));
+    VK_HIP(hipStreamSynchronize(s->st));
+    return VK_OK;
+}
+
+int vk_solver_step(vk_solver *s, int nsteps) {
+    if (int r = check(s)) return r;
+    for (int k = 0; k < nsteps; ++k)
+        if (int r = one_step(s, nullptr)) return r;
+    VK_HIP(hipStreamSynchronize(s->st));
+    return VK_OK;
+}
+
+int vk_solver_step_timed(vk_solver *s, int nsteps, vk_timing *t) {
+    if (int r = check(s)) return r;
+    if (!t) return fail(VK_ERR_ARG, "null timing pointer");
+    hipEvent_t t0, t1;
+    VK_HIP(hipEventCreate(&t0));
+    VK_HIP(hipEventCreate(&t1));
+    VK_HIP(hipEventRecord(t0, s->st));
+    for (int k = 0; k < nsteps; ++k)
+        if (int r = one_step(s, t)) return r;
+    VK_HIP(hipEventRecord(t1, s->st));
+    VK_HIP(hipEventSynchronize(t1));
+    float ms = 0;
+    VK_HIP(hipEventElapsedTime(&ms, t0, t1));
+    t->total += ms;
+    t->steps += nsteps;
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    return VK_OK;
+}
+
+int vk_solver_time(vk_solver *s, double *v) {
+    if (int r = check(s)) return r;
+    Scalars h;
+    if (int r = read_scalars(s, h)) return r;
+    *v = h.time;
+    return VK_OK;
+}
+int vk_solver_dt(vk_solver *s, double *v) {
+    if (int r = check(s)) return r;
+    Scalars h;
+    if (int r = read_scalars(s, h)) return r;
+    *v = h.dt;
+    return VK_OK;
+}
+int vk_solver_step_count(vk_solver *s, uint64_t *v) {
+    if (int r = check(s)) return r;
+    Scalars h;
+    if (int r = read_scalars(s, h)) return r;
+    *v = h.step;
+    return VK_OK;
+}
+
+static int diag_sums(vk_solver *s, double *spec_out) {
+    VK_TRY(launch_diag(static_cast<const double *>(s->u.p), s->n, s->hp, s->S, s->ks, s->nshells,
+                       static_cast<double *>(s->partials.p), s->nblocks_diag, spec_out, s->dsc(), s->st));
+    return 0;
+}
+
+int vk_solver_diagnostics(vk_solver *s, vk_diag *d) {
+    if (int r = check(s)) return r;
+    if (!d) return fail(VK_ERR_ARG, "null diag pointer");
+    if (int r = diag_sums(s, nullptr)) return r;
+    if (int r = max_pass(s, true)) return r;
+    Scalars h;
+    if (int r = read_scalars(s, h)) return r;
+    const double nu = s->nu;
+    const double max_vort = std::sqrt(bits_to_double(h.umax2_bits));
+    d->step = h.step;
+    d->time = h.time;
+    d->dt = h.dt;
+    d->energy = h.diag[0];
+    d->enstrophy = h.diag[1];
+    d->helicity = h.diag[2];
+    d->superhelicity = h.diag[3];
+    d->max_vorticity = max_vort;
+    // ScalarDiagnostics::compute_rates (scalar.rs:52-54) and cfl (scalar.rs:29-31)
+    d->energy_dissipation_rate = -2.0 * nu * h.diag[1];
+    d->helicity_dissipation_rate = -2.0 * nu * h.diag[3];
+    d->cfl_number = max_vort * h.dt / s->dx;
+    return VK_OK;
+}
+
+static int scalar_diag(vk_solver *s, double *v, int which) {
+    if (int r = check(s)) return r;
+    if (!v) return fail(VK_ERR_ARG, "null pointer");
+    if (int r = diag_sums(s, nullptr)) return r;
+    Scalars h;
+    if (int r = read_scalars(s, h)) return r;
+    *v = h.diag[which];
+    return VK_OK;
+}
+int vk_solver_energy(vk_solver *s, double *v) { return scalar_diag(s, v, 0); }
+int vk_solver_enstrophy(vk_solver *s, double *v) { return scalar_diag(s, v, 1); }
+int vk_solver_helicity(vk_solver *s, double *v) { return scalar_diag(s, v, 2); }
+int vk_solver_superhelicity(vk_solver *s, double *v) { return scalar_diag(s, v, 3); }
+
+int vk_solver_max_vorticity(vk_solver *s, double *v) {
+    if (int r = check(s)) return r;
+    if (!v) return fail(VK_ERR_ARG, "null pointer");
+    if (int r = max_pass(s, true)) return r;
+    Scalars h;
+    if (int r = read_scalars(s, h)) return r;
+    *v = std::sqrt(bits_to_double(h.umax2_bits));
+    return VK_OK;
+}
+
+int vk_solver_cfl_dt(vk_solver *s, double *v) {
+    if (int r = check(s)) return r;
+    if (!v) return fail(VK_ERR_ARG, "null pointer");
+    if (int r = max_pass(s, false)) return r;
+    Scalars h;
+    if (int r = read_scalars(s, h)) return r;
+    // compute_cfl_dt_static (solver.rs:180-218)
+    const double u_max = std::sqrt(bits_to_double(h.umax2_bits));
+    const double advective = u_max > 1e-30 ? s->dx / u_max : 1.0;
+    const double viscous = s->nu > 1e-30 ? s->dx * s->dx / s->nu : INFINITY;
+    *v = 0.5 * std::fmin(std::fmin(advective, viscous), 0.1);
+    return VK_OK;
+}
+
+int vk_solver_energy_spectrum(vk_solver *s, double *out, int cap, int *n_shells) {
+    if (int r = check(s)) return r;
+    if (int r = diag_sums(s, static_cast<double *>(s->spec.p))) return r;
+    const int k = std::min(cap, s->nshells);
+    if (out && k > 0) VK_HIP(hipMemcpyAsync(out, s->spec.p, (size_t)k * sizeof(double), hipMemcpyDeviceToHost, s->st));
+    VK_HIP(hipStreamSynchronize(s->st));
+    if (n_shells) *n_shells = s->nshells;
+    return VK_OK;
+}
+
+int vk_solver_nonlinear(vk_solver *s, const double *uhat, double *nhat) {
+    if (int r = check(s)) return r;
+    if (!uhat || !nhat) return fail(VK_ERR_ARG, "null pointer");
+    const size_t bytes = (size_t)3 * s->n * s->n * s->h * 16;
+    VK_HIP(hipMemcpyAsync(s->staging.p, uhat, bytes, hipMemcpyHostToDevice, s->st));
+    VK_TRY(launch_repack(s->staging.p, s->x.p, (long long)3 * s->n * s->n, s->h, s->hp, s->h, 1, s->st));
+    StageArgs sa = stage_args(s, ST_RAW, s->n1.p);
+    if (int r = nl_eval(s, s->x.p, sa, nullptr, nullptr)) return r;
+    return get_state_impl(s, nhat, false, s->n1.p);
+}
+
+int vk_solver_physical(vk_solver *s, double *u_phys, double *w_phys) {
+    if (int r = check(s)) return r;
+    // transform u (and curl u) to physical space through a standalone plan
+    vk_fft_plan *p = nullptr;
+    if (int r = vk_fft_plan_create(s->n, s->n, s->n, &p)) return r;
+    const size_t P = (size_t)s->n * s->n * s->n;
+    const size_t H = (size_t)s->n * s->n * s->h;
+    std::vector<double> uh(3 * H * 2);
+    int r = get_state_impl(s, uh.data(), false);
+    if (!r && u_phys)
+        for (int c = 0; c < 3 && !r; ++c) r = vk_fft_c2r_3d(p, uh.data() + c * H * 2, u_phys + c * P);
+    if (!r && w_phys) {
+        // SpectralOps::curl on the host copy (spectral_ops.rs:81-108)
+        std::vector<double> wh(3 * H * 2);
+        const int n = s->n, h = s->h;
+        for (int ix = 0; ix < n; ++ix)
+            for (int iy = 0; iy < n; ++iy)
+                for (int iz = 0; iz < h; ++iz) {
+                    const double kx = (ix < n / 2 ? ix : ix - n) * s->ks, ky = (iy < n / 2 ? iy : iy - n) * s->ks,
+                                 kz = iz * s->ks;
+                    const size_t i = ((size_t)ix * n + iy) * h + iz;
+                    const double *ux = &uh[2 * i], *uy = &uh[2 * (H + i)], *uz = &uh[2 * (2 * H + i)];
+                    double *wx = &wh[2 * i], *wy = &wh[2 * (H + i)], *wz = &wh[2 * (2 * H + i)];
+                    wx[0] = -(ky * uz[1] - kz * uy[1]);
+                    wx[1] = ky * uz[0] - kz * uy[0];
+                    wy[0] = -(kz * ux[1] - kx * uz[1]);
+                    wy[1] = kz * ux[0] - kx * uz[0];
+                    wz[0] = -(kx * uy[1] - ky * ux[1]);
+                    wz[1] = kx * uy[0] - ky * ux[0];
+                }
+        for (int c = 0; c < 3 && !r; ++c) r = vk_fft_c2r_3d(p, wh.data() + c * H * 2, w_phys + c * P);
+    }
+    std::string msg = g_err;
+    vk_fft_plan_destroy(p);
+    g_err = msg;
+    return r;
+}
+
+int vk_solver_device_bytes(const vk_solver *s, uint64_t *bytes) {
+    if (!s || !bytes) return fail(VK_ERR_ARG, "null pointer");
+    *bytes = s->total;
+    return VK_OK;
+}
+
+// taylor_green (vonkarman-periodic/src/ic/taylor_green.rs:10-38)
+int vk_ic_taylor_green(int n, double l, double *u) {
+    if (n < 1 || !u) return fail(VK_ERR_ARG, "vk_ic_taylor_green: bad argument");
+    const size_t P = (size_t)n * n * n;
+    const double fn = (double)n;
+    for (int i = 0; i < n; ++i) {
+        const double x = (double)i * l / fn;
+        const double sx = std::sin(x), cx = std::cos(x);
+        for (int j = 0; j < n; ++j) {
+            const double y = (double)j * l / fn;
+            const double sy = std::sin(y), cy = std::cos(y);
+            for (int k = 0; k < n; ++k) {
+                const double z = (double)k * l / fn;
+                const double cz = std::cos(z);
+                const size_t idx = ((size_t)i * n + j) * n + k;
+                u[idx] = sx * cy * cz;
+                u[P + idx] = -(cx * sy * cz);
+                u[2 * P + idx] = 0.0;
+            }
+        }
+    }
+    return VK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,291 @@
+// vk_common.hip -- length-independent kernels (ETD table, CFL update, clock,
+// diagnostics reductions, row repacking) and the per-length dispatchers.
+#include <cmath>
+
+#include "vk_fft.hpp"
+#include "vk_launch.h"
+#include "vk_lset.h"
+
+namespace vk {
+
+#define VK_CASE_INV(LEN) \
+    case LEN: return launch_inv_L<LEN>(mode, lin, ip, wp, in, out, tw, g, n, ks, scale, st);
+int launch_inv(int mode, int L, int lin, Prec ip, Prec wp, const void *in, void *out, const void *tw,
+               const Strided &g, int n, double ks, double scale, hipStream_t st) {
+    switch (L) { VK_FOR_EACH_LENGTH(VK_CASE_INV) }
+    return 2;
+}
+#define VK_CASE_FWD(LEN) \
+    case LEN: return launch_fwd_L<LEN>(mode, lout, nf, wp, in, out, tw, g, n, ks, scale, sa, st);
+int launch_fwd(int mode, int L, int lout, int nf, Prec wp, const void *in, void *out, const void *tw,
+               const Strided &g, int n, double ks, double scale, const StageArgs *sa, hipStream_t st) {
+    switch (L) { VK_FOR_EACH_LENGTH(VK_CASE_FWD) }
+    return 2;
+}
+#define VK_CASE_ZNL(LEN) \
+    case LEN: return launch_z_nl_L<LEN>(wp, in, out, lines, hp, fs_in, fs_out, s_in, s_out, tw, st);
+int launch_z_nl(int L, Prec wp, const void *in, void *out, long long lines, int hp, long long fs_in,
+                long long fs_out, double s_in, double s_out, const void *tw, hipStream_t st) {
+    switch (L) { VK_FOR_EACH_LENGTH(VK_CASE_ZNL) }
+    return 2;
+}
+#define VK_CASE_ZMAX(LEN) \
+    case LEN: return launch_z_max_L<LEN>(wp, in, lines, hp, fs, s_in, tw, sc, st);
+int launch_z_max(int L, Prec wp, const void *in, long long lines, int hp, long long fs, double s_in,
+                 const void *tw, Scalars *sc, hipStream_t st) {
+    switch (L) { VK_FOR_EACH_LENGTH(VK_CASE_ZMAX) }
+    return 2;
+}
+#define VK_CASE_ZC2R(LEN) \
+    case LEN: return launch_z_c2r_L<LEN>(in, out, lines, hp, scale, tw, st);
+int launch_z_c2r(int L, Prec prec, const void *in, double *out, long long lines, int hp, double scale,
+                 const void *tw, hipStream_t st) {
+    if (prec != F64) return 2;
+    switch (L) { VK_FOR_EACH_LENGTH(VK_CASE_ZC2R) }
+    return 2;
+}
+#define VK_CASE_ZR2C(LEN) \
+    case LEN: return launch_z_r2c_L<LEN>(in, out, lines, hp, tw, st);
+int launch_z_r2c(int L, Prec prec, const double *in, void *out, long long lines, int hp, const void *tw,
+                 hipStream_t st) {
+    if (prec != F64) return 2;
+    switch (L) { VK_FOR_EACH_LENGTH(VK_CASE_ZR2C) }
+    return 2;
+}
+
+static int last_launch() { return hipGetLastError() == hipSuccess ? 0 : 3; }
+
+// ---------------------------------------------------------------------------
+// row repack: [nfields][rows][src_pitch] -> [nfields][rows][dst_pitch]
+// (complex double), copying `width` entries and zeroing the rest of each row
+// ---------------------------------------------------------------------------
+__global__ void k_repack(const cpx<double> *__restrict__ src, cpx<double> *__restrict__ dst, long long rows,
+                         int sp, int dp, int width, int nf) {
+    const long long total = (long long)nf * rows * dp;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int k = (int)(i % dp);
+        const long long r = i / dp;  // field*rows + row
+        dst[i] = k < width ? src[r * sp + k] : cpx<double>{0.0, 0.0};
+    }
+}
+int launch_repack(const void *src, void *dst, long long rows, int src_pitch, int dst_pitch, int width,
+                  int nfields, hipStream_t st) {
+    const long long total = (long long)nfields * rows * dst_pitch;
+    const unsigned nb = (unsigned)std::min<long long>((total + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_repack, dim3(nb), dim3(256), 0, st, reinterpret_cast<const cpx<double> *>(src),
+                       reinterpret_cast<cpx<double> *>(dst), rows, src_pitch, dst_pitch, width, nfields);
+    return last_launch();
+}
+
+__global__ void k_fill_u64(unsigned long long *p, unsigned long long v, long long n) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+int launch_fill_u64(unsigned long long *p, unsigned long long v, long long n, hipStream_t st) {
+    const unsigned nb = (unsigned)std::min<long long>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_fill_u64, dim3(nb), dim3(256), 0, st, p, v, n);
+    return last_launch();
+}
+
+// ---------------------------------------------------------------------------
+// CFL rule (vonkarman-periodic/src/solver.rs:180-218 and :383-395):
+//   dt_new = safety * min(dx/max|u| (1 if max|u| <= 1e-30), dx^2/nu (inf if nu <= 1e-30), 0.1)
+//   adopted when |dt_new - dt| / max(dt, 1e-30) > 0.01 (always when unconditional)
+// ---------------------------------------------------------------------------
+__global__ void k_cfl_update(Scalars *sc, double dx, double nu, double safety, int unconditional) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double u_max = sqrt(__longlong_as_double((long long)sc->umax2_bits));
+    const double advective = u_max > 1e-30 ? dx / u_max : 1.0;
+    const double viscous = nu > 1e-30 ? dx * dx / nu : INFINITY;
+    const double dt_new = safety * fmin(fmin(advective, viscous), 0.1);
+    const double dt = sc->dt;
+    if (unconditional || fabs(dt_new - dt) / fmax(dt, 1e-30) > 0.01) {
+        sc->dt = dt_new;
+        sc->rebuild = 1;
+    } else {
+        sc->rebuild = 0;
+    }
+}
+int launch_cfl_update(Scalars *sc, double dx, double nu, double safety, int unconditional, hipStream_t st) {
+    hipLaunchKernelGGL(k_cfl_update, dim3(1), dim3(64), 0, st, sc, dx, nu, safety, unconditional);
+    return last_launch();
+}
+
+__global__ void k_clock_advance(Scalars *sc) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    sc->time += sc->dt;
+    sc->step += 1;
+}
+int launch_clock_advance(Scalars *sc, hipStream_t st) {
+    hipLaunchKernelGGL(k_clock_advance, dim3(1), dim3(64), 0, st, sc);
+    return last_launch();
+}
+
+// ---------------------------------------------------------------------------
+// phi functions by the Kassam-Trefethen contour mean (32 points, radius 1)
+// and the ETD-RK4 coefficients (vonkarman-periodic/src/etd.rs:15-134), in FP64.
+// The table is indexed by the integer |k|^2 (k2 = kx^2+ky^2+kz^2 is an integer
+// for the 2 pi box), so every mode of a shell shares one entry.
+// ---------------------------------------------------------------------------
+__device__ void phi123(double z, double &p1, double &p2, double &p3) {
+    const double PI = 3.14159265358979323846;
+    const int M = 32;
+    double s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    for (int j = 0; j < M; ++j) {
+        const double theta = 2.0 * PI * ((double)j + 0.5) / (double)M;
+        const double w_re = z + 1.0 * cos(theta);
+        const double w_im = 1.0 * sin(theta);
+        const double ex = exp(w_re);
+        const double ew_re = ex * cos(w_im);
+        const double ew_im = ex * sin(w_im);
+        {  // phi1
+            const double num_re = ew_re - 1.0, num_im = ew_im;
+            const double denom = w_re * w_re + w_im * w_im;
+            s1 += (num_re * w_re + num_im * w_im) / denom;
+        }
+        const double w2_re = w_re * w_re - w_im * w_im;
+        const double w2_im = 2.0 * w_re * w_im;
+        {  // phi2
+            const double num_re = ew_re - 1.0 - w_re, num_im = ew_im - w_im;
+            const double denom = w2_re * w2_re + w2_im * w2_im;
+            s2 += (num_re * w2_re + num_im * w2_im) / denom;
+        }
+        {  // phi3
+            const double num_re = ew_re - 1.0 - w_re - 0.5 * w2_re;
+            const double num_im = ew_im - w_im - 0.5 * w2_im;
+            const double w3_re = w_re * w2_re - w_im * w2_im;
+            const double w3_im = w_re * w2_im + w_im * w2_re;
+            const double denom = w3_re * w3_re + w3_im * w3_im;
+            s3 += (num_re * w3_re + num_im * w3_im) / denom;
+        }
+    }
+    p1 = fabs(z) < 1e-300 ? 1.0 : s1 / (double)M;
+    p2 = fabs(z) < 1e-300 ? 0.5 : s2 / (double)M;
+    p3 = fabs(z) < 1e-300 ? 1.0 / 6.0 : s3 / (double)M;
+}
+
+__global__ void k_etd_table(double *tab, int qmax, double nu, double ks, const Scalars *sc, int force) {
+    if (!force && !sc->rebuild) return;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q > qmax) return;
+    const int S = qmax + 1;
+    const double k2 = (double)q * (ks * ks);
+    const double lambda = -nu * k2;
+    const double z = lambda * sc->dt;
+    const double zh = z / 2.0;
+    double p1h, p2h, p3h, p1, p2, p3;
+    phi123(zh, p1h, p2h, p3h);
+    phi123(z, p1, p2, p3);
+    tab[0 * S + q] = exp(z);
+    tab[1 * S + q] = exp(zh);
+    tab[2 * S + q] = p1h;
+    tab[3 * S + q] = p1;
+    tab[4 * S + q] = p1 - 3.0 * p2 + 4.0 * p3;
+    tab[5 * S + q] = 2.0 * p2 - 4.0 * p3;
+    tab[6 * S + q] = -p2 + 4.0 * p3;
+}
+int launch_etd_table(double *tab, int qmax, double nu, double ks, const Scalars *sc, int force, hipStream_t st) {
+    const unsigned nb = (unsigned)((qmax + 1 + 127) / 128);
+    hipLaunchKernelGGL(k_etd_table, dim3(nb), dim3(128), 0, st, tab, qmax, nu, ks, sc, force);
+    return last_launch();
+}
+
+// ---------------------------------------------------------------------------
+// Diagnostics (vonkarman-periodic/src/solver.rs:412-535, vonkarman-diag/src/
+// spectrum.rs:13-60): one pass over u computes the R2C-weighted sums of
+// |u|^2, |w|^2, u.w, w.curl(w) and the shell histogram of |u|^2.  Each thread
+// keeps Neumaier-compensated sums; blocks reduce in a fixed tree order and a
+// final block sums the per-block partials in fixed order (compensated).
+// ---------------------------------------------------------------------------
+struct Neu {
+    double s = 0.0, c = 0.0;
+    __device__ void add(double x) {
+        const double t = s + x;
+        c += fabs(s) >= fabs(x) ? (s - t) + x : (x - t) + s;
+        s = t;
+    }
+    __device__ double val() const { return s + c; }
+};
+
+constexpr int DIAG_NT = 256;
+constexpr int DIAG_MAXSH = 1024;
+
+__device__ __forceinline__ void curl_mode(double kx, double ky, double kz, const cpx<double> u[3], cpx<double> w[3]) {
+    // SpectralOps::curl (vonkarman-core/src/spectral_ops.rs:93-104)
+    w[0] = {-(ky * u[2].im - kz * u[1].im), ky * u[2].re - kz * u[1].re};
+    w[1] = {-(kz * u[0].im - kx * u[2].im), kz * u[0].re - kx * u[2].re};
+    w[2] = {-(kx * u[1].im - ky * u[0].im), kx * u[1].re - ky * u[0].re};
+}
+
+__global__ __launch_bounds__(DIAG_NT) void k_diag(const cpx<double> *__restrict__ u, int n, int hp, long long fs,
+                                                 double ks, int nshells, double *__restrict__ partials) {
+    __shared__ double spec[DIAG_MAXSH];
+    __shared__ double red[4][DIAG_NT];
+    const int tid = threadIdx.x;
+    for (int s = tid; s < nshells; s += DIAG_NT) spec[s] = 0.0;
+    __syncthreads();
+    const int h = n / 2 + 1;
+    const long long total = (long long)n * n * hp;
+    Neu acc[4];
+    for (long long i = blockIdx.x * (long long)DIAG_NT + tid; i < total; i += (long long)gridDim.x * DIAG_NT) {
+        const int iz = (int)(i % hp);
+        if (iz >= h) continue;
+        const long long r = i / hp;
+        const int iy = (int)(r % n), ix = (int)(r / n);
+        const double kx = double(wavenum(ix, n)) * ks, ky = double(wavenum(iy, n)) * ks, kz = double(iz) * ks;
+        const cpx<double> uu[3] = {u[i], u[fs + i], u[2 * fs + i]};
+        cpx<double> w[3], cw[3];
+        curl_mode(kx, ky, kz, uu, w);
+        curl_mode(kx, ky, kz, w, cw);
+        const double wt = (iz == 0 || iz == n / 2) ? 1.0 : 2.0;
+        double e = 0.0, o = 0.0, hh = 0.0, ss = 0.0;
+        for (int c = 0; c < 3; ++c) {
+            e += uu[c].re * uu[c].re + uu[c].im * uu[c].im;
+            o += w[c].re * w[c].re + w[c].im * w[c].im;
+            hh += uu[c].re * w[c].re + uu[c].im * w[c].im;
+            ss += w[c].re * cw[c].re + w[c].im * cw[c].im;
+        }
+        acc[0].add(wt * e);
+        acc[1].add(wt * o);
+        acc[2].add(wt * hh);
+        acc[3].add(wt * ss);
+        const int shell = (int)rint(sqrt(kx * kx + ky * ky + kz * kz));
+        if (shell < nshells) atomicAdd(&spec[shell], 0.5 * wt * e);
+    }
+    for (int k = 0; k < 4; ++k) red[k][tid] = acc[k].val();
+    __syncthreads();
+    for (int off = DIAG_NT / 2; off > 0; off >>= 1) {
+        if (tid < off)
+            for (int k = 0; k < 4; ++k) red[k][tid] += red[k][tid + off];
+        __syncthreads();
+    }
+    double *row = partials + (long long)blockIdx.x * (4 + nshells);
+    if (tid < 4) row[tid] = red[tid][0];
+    for (int s = tid; s < nshells; s += DIAG_NT) row[4 + s] = spec[s];
+}
+
+__global__ void k_diag_final(const double *__restrict__ partials, int nblocks, int nshells, double inv_nt2,
+                             double *__restrict__ spec_out, Scalars *sc) {
+    const int ncol = 4 + nshells;
+    for (int c = threadIdx.x; c < ncol; c += blockDim.x) {
+        Neu a;
+        for (int b = 0; b < nblocks; ++b) a.add(partials[(long long)b * ncol + c]);
+        const double v = a.val() * inv_nt2;
+        if (c < 4) sc->diag[c] = c == 0 ? 0.5 * v : v;
+        else if (spec_out) spec_out[c - 4] = v;
+    }
+}
+
+int launch_diag(const double *u, int n, int hp, long long fs, double ks, int nshells, double *partials, int nblocks,
+                double *spec_out, Scalars *sc, hipStream_t st) {
+    if (nshells > DIAG_MAXSH) return 2;
+    hipLaunchKernelGGL(k_diag, dim3(nblocks), dim3(DIAG_NT), 0, st, reinterpret_cast<const cpx<double> *>(u), n, hp,
+                       fs, ks, nshells, partials);
+    const double nt = (double)n * n * n;
+    hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(256), 0, st, partials, nblocks, nshells, 1.0 / (nt * nt), spec_out,
+                       sc);
+    return last_launch();
+}
+
+}  // namespace vk

@@ -1,0 +1,98 @@
+// vk_launch.h -- host-side launchers for the gfx950 kernels (internal to the
+// library; the public C ABI is include/vonkarman_hip.h).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace vk {
+
+enum Prec { F64 = 0, F32 = 1 };
+
+// Geometry of a strided (x or y) pass.  A "column" is one line of the pass;
+// position p of column col in field f lives at
+//   base + f*fs + (col / inner)*os + (col % inner) + p*ls      (complex elements)
+// Columns with (col % hp) >= hvalid are row padding and are neither read nor
+// written.
+struct Strided {
+    int ncols;
+    int hp, hvalid;
+    int in_inner;
+    long long in_os, in_ls, in_fs;
+    int out_inner;
+    long long out_os, out_ls, out_fs;
+};
+
+// Device scalars of one solver (one allocation, read by kernels in place so a
+// time step never synchronises with the host).
+struct Scalars {
+    double dt;
+    double time;
+    unsigned long long umax2_bits;  // max |v|^2 of the last max pass (bit pattern)
+    unsigned long long step;
+    int rebuild;                     // ETD table must be rebuilt (dt changed)
+    int pad_;
+    double diag[8];                  // E, Omega, H, S (reduced)
+};
+
+// inverse strided passes ---------------------------------------------------
+enum InvMode {
+    INV_PLAIN = 0,   // 1 field -> 1 field
+    INV_U3 = 1,      // 3 -> 3
+    INV_U3DX = 2,    // x pass: (ux,uy,uz) -> (ux,uy,uz, i kx uz, i kx uy)
+    INV_OMEGA = 3,   // y pass: 5 -> (wx,wy,wz)
+    INV_UOMEGA = 4,  // y pass: 5 -> (ux,uy,uz,wx,wy,wz)
+};
+// lin: input length along the line (n for padded NL passes, L otherwise)
+// n: unpadded grid size (wavenumbers), ks: 2 pi / box length
+int launch_inv(int mode, int L, int lin, Prec in_prec, Prec work_prec, const void *in, void *out,
+               const void *tw, const Strided &g, int n, double ks, double scale, hipStream_t st);
+
+// forward strided passes ----------------------------------------------------
+enum FwdMode {
+    FWD_PLAIN = 0,  // nf fields, each truncated to lout positions
+    FWD_STAGE = 1,  // 3 fields, truncate, Leray, then an ETD-RK4 stage epilogue
+};
+enum Stage { ST_RAW = 0, ST_1 = 1, ST_2 = 2, ST_3 = 3, ST_4 = 4 };
+struct StageArgs {
+    int stage;
+    void *out;        // ST_RAW: N hat; ST_1..3: next stage input; ST_4: u (in place)
+    const void *u;    // state
+    void *n1;         // N1 (written by ST_1, read by ST_3/ST_4)
+    void *sacc;       // N2+N3 (written by ST_2/ST_3, read by ST_4)
+    long long fs;     // field stride of the state-layout arrays
+    const double *tab;  // ETD table, 7 arrays of (qmax+1) doubles
+    int tab_stride;     // qmax+1
+    const Scalars *sc;  // dt
+};
+int launch_fwd(int mode, int L, int lout, int nf, Prec work_prec, const void *in, void *out,
+               const void *tw, const Strided &g, int n, double ks, double scale,
+               const StageArgs *sa, hipStream_t st);
+
+// contiguous (z) passes -------------------------------------------------------
+// NL z pass: 6 padded half-spectra (ux,uy,uz,wx,wy,wz) -> 3 truncated spectra of u x w
+int launch_z_nl(int L, Prec work_prec, const void *in, void *out, long long lines, int hp,
+                long long fs_in, long long fs_out, double s_in, double s_out, const void *tw,
+                hipStream_t st);
+// C2R of 3 unpadded fields and max |v|^2 into sc->umax2_bits
+int launch_z_max(int L, Prec work_prec, const void *in, long long lines, int hp, long long fs,
+                 double s_in, const void *tw, Scalars *sc, hipStream_t st);
+// standalone single-field C2R / R2C along the contiguous axis
+int launch_z_c2r(int L, Prec prec, const void *in, double *out, long long lines, int hp,
+                 double scale, const void *tw, hipStream_t st);
+int launch_z_r2c(int L, Prec prec, const double *in, void *out, long long lines, int hp,
+                 const void *tw, hipStream_t st);
+
+// helpers ---------------------------------------------------------------------
+int launch_repack(const void *src, void *dst, long long rows, int src_pitch, int dst_pitch,
+                  int width, int nfields, hipStream_t st);
+int launch_fill_u64(unsigned long long *p, unsigned long long v, long long n, hipStream_t st);
+int launch_cfl_update(Scalars *sc, double dx, double nu, double safety, int unconditional,
+                      hipStream_t st);
+int launch_clock_advance(Scalars *sc, hipStream_t st);
+int launch_etd_table(double *tab, int qmax, double nu, double ks, const Scalars *sc,
+                     int force, hipStream_t st);
+// diagnostics: E, Omega, H, S and the shell spectrum in one pass over u
+int launch_diag(const double *u, int n, int hp, long long fs, double ks, int nshells,
+                double *partials, int nblocks, double *spec_out, Scalars *sc, hipStream_t st);
+
+}  // namespace vk

@@ -1,0 +1,128 @@
+"""Initial conditions.
+
+``taylor_green`` restates vonkarman-periodic/src/ic/taylor_green.rs:10-38 (the
+physical field is computed by the C ABI's host routine, glibc sin/cos, exactly
+as the reference evaluates it).  ``von_karman_isotropic`` and
+``antiparallel_tubes`` are the workload initial fields of this backend's
+benchmark configurations (the reference ships neither; its closest are
+ic/random_isotropic.rs and ic/anti_parallel.rs).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from ._lib import check, lib, ptr
+
+
+def taylor_green(n: int, l: float = 2.0 * math.pi) -> np.ndarray:
+    """u = (sin x cos y cos z, -cos x sin y cos z, 0) on the n^3 grid x_i = i l/n."""
+    out = np.empty((3, n, n, n), np.float64)
+    check(lib().vk_ic_taylor_green(n, l, ptr(out)), "vk_ic_taylor_green")
+    return out
+
+
+def _wavenumbers(n: int):
+    k = np.fft.fftfreq(n, 1.0 / n)
+    k[n // 2] = -(n // 2)  # index n/2 -> -n/2 (spectral_ops.rs:55-61)
+    kz = np.arange(n // 2 + 1, dtype=np.float64)
+    return k, k.copy(), kz
+
+
+def _leray(uh: np.ndarray, kx, ky, kz) -> np.ndarray:
+    KX, KY, KZ = np.meshgrid(kx, ky, kz, indexing="ij")
+    k2 = KX * KX + KY * KY + KZ * KZ
+    inv = np.where(k2 < 1e-30, 0.0, 1.0 / np.where(k2 < 1e-30, 1.0, k2))
+    kdu = KX * uh[0] + KY * uh[1] + KZ * uh[2]
+    return np.stack([uh[0] - KX * kdu * inv, uh[1] - KY * kdu * inv, uh[2] - KZ * kdu * inv])
+
+
+def _energy(uh: np.ndarray, n: int) -> float:
+    w = np.full(n // 2 + 1, 2.0)
+    w[0] = 1.0
+    w[n // 2] = 1.0
+    nt = float(n) ** 3
+    return 0.5 * float(np.sum(w * (uh.real ** 2 + uh.imag ** 2))) / (nt * nt)
+
+
+def _hermitian_plane(p: np.ndarray) -> np.ndarray:
+    n = p.shape[0]
+    neg = (-np.arange(n)) % n
+    return 0.5 * (p + np.conj(p[np.ix_(neg, neg)]))
+
+
+def von_karman_isotropic(n: int, k0: float = 4.0, kd: float = 64.0, energy: float = 0.5,
+                         seed: int = 42) -> np.ndarray:
+    """Random isotropic spectral velocity with the von Karman spectrum
+    E(k) = (k/k0)^4 / (1+(k/k0)^2)^(17/6) * exp(-2 (k/kd)^2).
+
+    Per mode k != 0 and component, amplitude sqrt(E(|k|)) with a uniform random
+    phase (numpy PCG64 seeded with `seed`, drawn in C order of (c, ix, iy, iz));
+    then Leray projection, Hermitian symmetrisation of the kz=0 plane, the
+    Nyquist modes (any index n/2) set to zero, and scaling to total energy
+    `energy` under the R2C-weighted norm of solver.rs:412-434.
+    Returns complex128[3, n, n, n//2+1].
+    """
+    kx, ky, kz = _wavenumbers(n)
+    KX, KY, KZ = np.meshgrid(kx, ky, kz, indexing="ij")
+    k = np.sqrt(KX * KX + KY * KY + KZ * KZ)
+    r = k / k0
+    spec = r ** 4 / (1.0 + r * r) ** (17.0 / 6.0) * np.exp(-2.0 * (k / kd) ** 2)
+    amp = np.sqrt(spec)
+    amp[0, 0, 0] = 0.0
+    rng = np.random.Generator(np.random.PCG64(seed))
+    phase = 2.0 * np.pi * rng.random((3,) + k.shape)
+    uh = amp[None] * np.exp(1j * phase)
+    uh = _leray(uh, kx, ky, kz)
+    for c in range(3):
+        uh[c, :, :, 0] = _hermitian_plane(uh[c, :, :, 0])
+    h = n // 2
+    uh[:, h, :, :] = 0.0
+    uh[:, :, h, :] = 0.0
+    uh[:, :, :, h] = 0.0
+    e = _energy(uh, n)
+    if e > 1e-30:
+        uh *= math.sqrt(energy / e)
+    return np.ascontiguousarray(uh)
+
+
+def antiparallel_tubes(n: int, core_radius: float = 0.3, circulation: float = 1.0,
+                       offset: float = 0.8, amplitude: float = 0.1) -> np.ndarray:
+    """Two anti-parallel Lamb-Oseen vortex tubes along x (Gaussian vorticity
+    profile, circulation +-Gamma, core radius a) centred at y = pi +- offset,
+    z = pi, with the centreline displaced by amplitude*cos(x) in y.  The
+    vorticity omega_x(x, y, z) = sum_tubes +-Gamma/(pi a^2) exp(-r^2/a^2) (with
+    periodic images in y and z) is transformed, Leray-projected and converted
+    to velocity u_hat = i k x omega_hat / |k|^2.  Returns complex128[3, n, n, n//2+1]."""
+    L = 2.0 * math.pi
+    x = np.arange(n) * L / n
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    wx = np.zeros((n, n, n))
+    for sign, yc in ((1.0, math.pi + offset), (-1.0, math.pi - offset)):
+        yc_x = yc + amplitude * np.cos(X)
+        for iy in (-1, 0, 1):
+            for iz in (-1, 0, 1):
+                dy = Y - yc_x + iy * L
+                dz = Z - math.pi + iz * L
+                wx += sign * circulation / (math.pi * core_radius ** 2) * np.exp(-(dy * dy + dz * dz) / core_radius ** 2)
+    from .fft import HipFftBackend
+
+    wh = np.zeros((3, n, n, n // 2 + 1), np.complex128)
+    HipFftBackend(n, n, n).r2c_3d(np.ascontiguousarray(wx), wh[0])
+    kx, ky, kz = _wavenumbers(n)
+    wh = _leray(wh, kx, ky, kz)
+    KX, KY, KZ = np.meshgrid(kx, ky, kz, indexing="ij")
+    k2 = KX * KX + KY * KY + KZ * KZ
+    inv = np.where(k2 < 1e-30, 0.0, 1.0 / np.where(k2 < 1e-30, 1.0, k2))
+    # u_hat = i k x w_hat / k^2  (curl(u) = w for divergence-free u)
+    uh = np.stack([
+        1j * (KY * wh[2] - KZ * wh[1]) * inv,
+        1j * (KZ * wh[0] - KX * wh[2]) * inv,
+        1j * (KX * wh[1] - KY * wh[0]) * inv,
+    ])
+    h = n // 2
+    uh[:, h, :, :] = 0.0
+    uh[:, :, h, :] = 0.0
+    uh[:, :, :, h] = 0.0
+    return np.ascontiguousarray(uh)
