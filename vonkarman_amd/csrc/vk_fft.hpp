@@ -135,6 +135,9 @@ VK_HD int bfly_out(int j, int m) { return (j / NS) * NS * RAD + (j % NS) + m * N
 template <int L, int RAD, int NS>
 VK_HD int bfly_tw(int j, int m) { return m * (j % NS) * (L / (NS * RAD)); }
 
+template <int... R>
+struct RadixList {};
+
 #if defined(__HIPCC__)
 template <int L, int RAD, int NS, int D, int T, int NT, int SP, int SC, typename R>
 __device__ __forceinline__ void stockham_pass(cpx<R> *lds, const cpx<R> *__restrict__ tw, int tid) {
@@ -178,29 +181,23 @@ __device__ __forceinline__ void stockham_run(cpx<R> *lds, const cpx<R> *__restri
     stockham_pass<L, RAD, NS, D, T, NT, SP, SC>(lds, tw, tid);
     if constexpr (sizeof...(REST) > 0) stockham_run<L, NS * RAD, D, T, NT, SP, SC, R, REST...>(lds, tw, tid);
 }
+
+template <int L, int D, int T, int NT, int SP, int SC, typename R, int... RS>
+__device__ __forceinline__ void stockham_list(RadixList<RS...>, cpx<R> *lds, const cpx<R> *__restrict__ tw,
+                                              int tid) {
+    stockham_run<L, 1, D, T, NT, SP, SC, R, RS...>(lds, tw, tid);
+}
 #endif
 
-// radix plans per length; run<...>() transforms the tile in place.
+// radix plans per length (radices applied in order, NS growing 1, R1, R1 R2...)
 template <int L>
 struct Plan;
 
-#if defined(__HIPCC__)
-#define VK_PLAN(LEN, ...)                                                                   \
-    template <>                                                                             \
-    struct Plan<LEN> {                                                                      \
-        static constexpr int radices[] = {__VA_ARGS__};                                     \
-        template <int D, int T, int NT, int SP, int SC, typename R>                         \
-        static __device__ __forceinline__ void run(cpx<R> *lds, const cpx<R> *tw, int tid) { \
-            stockham_run<LEN, 1, D, T, NT, SP, SC, R, __VA_ARGS__>(lds, tw, tid);           \
-        }                                                                                   \
+#define VK_PLAN(LEN, ...)                          \
+    template <>                                    \
+    struct Plan<LEN> {                             \
+        using radix_list = RadixList<__VA_ARGS__>; \
     };
-#else
-#define VK_PLAN(LEN, ...)                               \
-    template <>                                         \
-    struct Plan<LEN> {                                  \
-        static constexpr int radices[] = {__VA_ARGS__}; \
-    };
-#endif
 
 VK_PLAN(8, 8)
 VK_PLAN(12, 4, 3)
@@ -217,6 +214,15 @@ VK_PLAN(384, 8, 4, 4, 3)
 VK_PLAN(512, 8, 8, 8)
 VK_PLAN(768, 8, 8, 4, 3)
 #undef VK_PLAN
+
+#if defined(__HIPCC__)
+// Transform the tile in place: entry and exit are barrier-separated, data in
+// natural order at lds[p*SP + c*SC].
+template <int L, int D, int T, int NT, int SP, int SC, typename R>
+__device__ __forceinline__ void fft_tile(cpx<R> *lds, const cpx<R> *__restrict__ tw, int tid) {
+    stockham_list<L, D, T, NT, SP, SC>(typename Plan<L>::radix_list{}, lds, tw, tid);
+}
+#endif
 
 // ---------------------------------------------------------------------------
 // dealiasing index maps (the reference's zero_pad_spectral / truncate_spectral

@@ -82,7 +82,7 @@ __global__ __launch_bounds__((TileCfg<L, TW>::NT)) void k_inv(const cpx<TI> *__r
             }
         }
         __syncthreads();
-        Plan<L>::template run<+1, T, NT, T, 1>(lds, tw, tid);
+        fft_tile<L, +1, T, NT, T, 1>(lds, tw, tid);
         for (int e = tid; e < L * T; e += NT) {
             const int p = e / T, col = col0 + e % T;
             if (col < g.ncols && (col % g.hp) < g.hvalid)
@@ -215,7 +215,7 @@ __global__ __launch_bounds__((TileCfg<L, TW>::NT)) void k_fwd(const cpx<TW> *__r
             lds[p * T + e % T] = v;
         }
         __syncthreads();
-        Plan<L>::template run<-1, T, NT, T, 1>(lds, tw, tid);
+        fft_tile<L, -1, T, NT, T, 1>(lds, tw, tid);
 #pragma unroll
         for (int i = 0; i < EPT; ++i) {
             const int e = tid + i * NT;
@@ -316,7 +316,7 @@ __global__ __launch_bounds__((TileCfg<L, TW>::NTZ)) void k_z_nl(const cpx<TW> *_
         z_build_pair<L, HIN, TZ, LP>(lds, in + (2 * q) * fs_in, in + (2 * q + 1) * fs_in, line0, lines, hp,
                                      tid, NT);
         __syncthreads();
-        Plan<L>::template run<+1, TZ, NT, 1, LP>(lds, tw, tid);
+        fft_tile<L, +1, TZ, NT, 1, LP>(lds, tw, tid);
 #pragma unroll
         for (int i = 0; i < EPT; ++i) {
             const int e = tid + i * NT;
@@ -346,7 +346,7 @@ __global__ __launch_bounds__((TileCfg<L, TW>::NTZ)) void k_z_nl(const cpx<TW> *_
         if (NE % NT == 0 || e < NE) lds[(e / L) * LP + e % L] = {r[0][i], r[1][i]};
     }
     __syncthreads();
-    Plan<L>::template run<-1, TZ, NT, 1, LP>(lds, tw, tid);
+    fft_tile<L, -1, TZ, NT, 1, LP>(lds, tw, tid);
     for (int e = tid; e < TZ * HOUT; e += NT) {
         const int t = e / HOUT, k = e % HOUT;
         const long long line = line0 + t;
@@ -364,7 +364,7 @@ __global__ __launch_bounds__((TileCfg<L, TW>::NTZ)) void k_z_nl(const cpx<TW> *_
         if (NE % NT == 0 || e < NE) lds[(e / L) * LP + e % L] = {r[2][i], TW(0)};
     }
     __syncthreads();
-    Plan<L>::template run<-1, TZ, NT, 1, LP>(lds, tw, tid);
+    fft_tile<L, -1, TZ, NT, 1, LP>(lds, tw, tid);
     for (int e = tid; e < TZ * HOUT; e += NT) {
         const int t = e / HOUT, k = e % HOUT;
         const long long line = line0 + t;
@@ -392,7 +392,7 @@ __global__ __launch_bounds__((TileCfg<L, TW>::NTZ)) void k_z_max(const cpx<TW> *
         z_build_pair<L, HIN, TZ, LP>(lds, in + (2 * q) * fs, q == 0 ? in + fs : nullptr, line0, lines, hp, tid,
                                      NT);
         __syncthreads();
-        Plan<L>::template run<+1, TZ, NT, 1, LP>(lds, tw, tid);
+        fft_tile<L, +1, TZ, NT, 1, LP>(lds, tw, tid);
 #pragma unroll
         for (int i = 0; i < EPT; ++i) {
             const int e = tid + i * NT;
@@ -448,7 +448,7 @@ __global__ __launch_bounds__((TileCfg<L, TW>::NTZ)) void k_z_c2r(const cpx<TW> *
         }
     }
     __syncthreads();
-    Plan<L>::template run<+1, NS, NT, 1, LP>(lds, tw, tid);
+    fft_tile<L, +1, NS, NT, 1, LP>(lds, tw, tid);
     for (int e = tid; e < NS * L; e += NT) {
         const int s = e / L, nn = e % L;
         const long long la = line0 + 2 * s, lb = la + 1;
@@ -474,7 +474,7 @@ __global__ __launch_bounds__((TileCfg<L, TW>::NTZ)) void k_z_r2c(const double *_
         lds[s * LP + nn] = {la < lines ? TW(in[la * L + nn]) : TW(0), lb < lines ? TW(in[lb * L + nn]) : TW(0)};
     }
     __syncthreads();
-    Plan<L>::template run<-1, NS, NT, 1, LP>(lds, tw, tid);
+    fft_tile<L, -1, NS, NT, 1, LP>(lds, tw, tid);
     const TW h = TW(0.5);
     for (int e = tid; e < NS * HOUT; e += NT) {
         const int s = e / HOUT, k = e % HOUT;

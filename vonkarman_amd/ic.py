@@ -57,8 +57,9 @@ def von_karman_isotropic(n: int, k0: float = 4.0, kd: float = 64.0, energy: floa
     """Random isotropic spectral velocity with the von Karman spectrum
     E(k) = (k/k0)^4 / (1+(k/k0)^2)^(17/6) * exp(-2 (k/kd)^2).
 
-    Per mode k != 0 and component, amplitude sqrt(E(|k|)) with a uniform random
-    phase (numpy PCG64 seeded with `seed`, drawn in C order of (c, ix, iy, iz));
+    Per mode k != 0 and component, amplitude sqrt(E(|k|) / (4 pi |k|^2)) -- so
+    the shell-summed spectrum follows E(k) -- with a uniform random phase
+    (numpy PCG64 seeded with `seed`, drawn in C order of (c, ix, iy, iz));
     then Leray projection, Hermitian symmetrisation of the kz=0 plane, the
     Nyquist modes (any index n/2) set to zero, and scaling to total energy
     `energy` under the R2C-weighted norm of solver.rs:412-434.
@@ -69,7 +70,8 @@ def von_karman_isotropic(n: int, k0: float = 4.0, kd: float = 64.0, energy: floa
     k = np.sqrt(KX * KX + KY * KY + KZ * KZ)
     r = k / k0
     spec = r ** 4 / (1.0 + r * r) ** (17.0 / 6.0) * np.exp(-2.0 * (k / kd) ** 2)
-    amp = np.sqrt(spec)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        amp = np.sqrt(spec / (4.0 * np.pi * k * k))
     amp[0, 0, 0] = 0.0
     rng = np.random.Generator(np.random.PCG64(seed))
     phase = 2.0 * np.pi * rng.random((3,) + k.shape)
