@@ -69,6 +69,10 @@ def lib():
             "or_solver_energy_spectrum": (i, [P, P, i]),
             "or_energy_spectrum": (i, [i, d, P, P, i]),
             "or_taylor_green": (i, [i, d, P]),
+            "or_c2r_3d_T": (i, [i, i, i, P, P]),
+            "or_r2c_3d_T": (i, [i, i, i, P, P]),
+            "or_nonlinear_vjp": (i, [i, d, P, P, P]),
+            "or_solver_step_vjp": (i, [P, P, P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -134,6 +138,33 @@ def nonlinear(uhat: np.ndarray, l: float = TWO_PI) -> np.ndarray:
     uhat = np.ascontiguousarray(uhat, dtype=np.complex128)
     out = np.empty_like(uhat)
     lib().or_nonlinear(uhat.shape[1], l, _p(uhat), _p(out))
+    return out
+
+
+def c2r_3d_T(g: np.ndarray) -> np.ndarray:
+    """Transpose of c2r_3d under the real inner product."""
+    g = np.ascontiguousarray(g, dtype=np.float64)
+    nx, ny, nz = g.shape
+    out = np.empty((nx, ny, nz // 2 + 1), np.complex128)
+    lib().or_c2r_3d_T(nx, ny, nz, _p(g), _p(out))
+    return out
+
+
+def r2c_3d_T(y: np.ndarray, nz: int | None = None) -> np.ndarray:
+    """Transpose of r2c_3d under the real inner product."""
+    y = np.ascontiguousarray(y, dtype=np.complex128)
+    nx, ny, hz = y.shape
+    nz = 2 * (hz - 1) if nz is None else nz
+    out = np.empty((nx, ny, nz), np.float64)
+    lib().or_r2c_3d_T(nx, ny, nz, _p(y), _p(out))
+    return out
+
+
+def nonlinear_vjp(uhat: np.ndarray, lam: np.ndarray, l: float = TWO_PI) -> np.ndarray:
+    uhat = np.ascontiguousarray(uhat, dtype=np.complex128)
+    lam = np.ascontiguousarray(lam, dtype=np.complex128)
+    out = np.empty_like(uhat)
+    lib().or_nonlinear_vjp(uhat.shape[1], l, _p(uhat), _p(lam), _p(out))
     return out
 
 
@@ -250,6 +281,13 @@ class Solver:
         d = dict(zip(DIAG_FIELDS, out.tolist()))
         d["step"] = int(d["step"])
         return d
+
+    def step_vjp(self, lam_out: np.ndarray) -> np.ndarray:
+        """(d step / d u)^T lam_out at the current state (fixed dt)."""
+        lam_out = np.ascontiguousarray(lam_out, dtype=np.complex128)
+        out = np.empty_like(lam_out)
+        lib().or_solver_step_vjp(self._h, _p(lam_out), _p(out))
+        return out
 
     def energy_spectrum(self) -> np.ndarray:
         cap = 4 * self.n

@@ -835,3 +835,198 @@ extern "C" int or_taylor_green(int n, double l, double *u) {
     }
     return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Discrete adjoint (the reference has none): transposes of every operator  */
+/* under the real inner product <a,b> = sum Re(a)Re(b) + Im(a)Im(b) over    */
+/* the stored entries.  The step is a polynomial in u, so these transposes  */
+/* are exact and are checked by dot-product and finite-difference tests.    */
+/* ------------------------------------------------------------------------ */
+namespace {
+
+/* transpose of or_c2r_3d: complex[nx][ny][hz] <- real[nx][ny][nz] */
+void c2r_3d_T(int nx, int ny, int nz, const double *g, cd *out) {
+    const int hz = nz / 2 + 1;
+    const Plan1D &pz = plan_for(nz);
+#pragma omp parallel
+    {
+        std::vector<cd> line(nz), work(nz);
+#pragma omp for schedule(static)
+        for (long r = 0; r < (long)nx * ny; ++r) {
+            for (int q = 0; q < nz; ++q) line[q] = cd(g[r * nz + q], 0.0);
+            fft1d(pz, line.data(), work.data(), -1);
+            for (int k = 0; k < hz; ++k) {
+                const bool edge = (k == 0 || 2 * k == nz);
+                const double c = (edge ? 1.0 : 2.0) / nz;
+                out[r * hz + k] = cd(line[k].real() * c, edge ? 0.0 : line[k].imag() * c);
+            }
+        }
+    }
+    fft_axis(out, nx, ny, hz, 1, -1, 1.0 / ny);
+    fft_axis(out, nx, ny, hz, 0, -1, 1.0 / nx);
+}
+
+/* transpose of or_r2c_3d: real[nx][ny][nz] <- complex[nx][ny][hz] */
+void r2c_3d_T(int nx, int ny, int nz, const cd *y, double *out) {
+    const int hz = nz / 2 + 1;
+    std::vector<cd> tmp(y, y + (size_t)nx * ny * hz);
+    fft_axis(tmp.data(), nx, ny, hz, 0, +1, 1.0);
+    fft_axis(tmp.data(), nx, ny, hz, 1, +1, 1.0);
+    const Plan1D &pz = plan_for(nz);
+#pragma omp parallel
+    {
+        std::vector<cd> line(nz), work(nz);
+#pragma omp for schedule(static)
+        for (long r = 0; r < (long)nx * ny; ++r) {
+            for (int q = 0; q < nz; ++q) line[q] = 0.0;
+            for (int k = 0; k < hz; ++k) line[k] = tmp[r * hz + k];
+            fft1d(pz, line.data(), work.data(), +1);
+            for (int q = 0; q < nz; ++q) out[r * nz + q] = line[q].real();
+        }
+    }
+}
+
+/* transpose of compute_nonlinear at u, applied to the cotangent lam */
+void nonlinear_vjp(const Ops &o, const cd *u, const cd *lam, cd *g) {
+    const int n = o.n, m = 3 * n / 2;
+    const size_t S = o.sz(), P = (size_t)m * m * m, SP = (size_t)m * m * (m / 2 + 1);
+    const double scale = (double)(m * m * m) / (double)((long)n * n * n);
+    const double inv_scale = 1.0 / scale;
+    // forward recompute of the physical u and w
+    std::vector<cd> w(3 * S);
+    curl(o, u, w.data());
+    std::vector<double> U(3 * P), W(3 * P), gc(3 * P);
+    std::vector<cd> pad(SP);
+    for (int c = 0; c < 3; ++c) {
+        zero_pad(u + c * S, pad.data(), n, m);
+        for (auto &v : pad) v = cd(v.real() * scale, v.imag() * scale);
+        or_c2r_3d(m, m, m, reinterpret_cast<double *>(pad.data()), U.data() + c * P);
+        zero_pad(w.data() + c * S, pad.data(), n, m);
+        for (auto &v : pad) v = cd(v.real() * scale, v.imag() * scale);
+        or_c2r_3d(m, m, m, reinterpret_cast<double *>(pad.data()), W.data() + c * P);
+    }
+    // cotangent of u x w in physical space: R2C^T pad (inv_scale P lam)
+    std::vector<cd> l(lam, lam + 3 * S);
+    leray(o, l.data());
+    for (int c = 0; c < 3; ++c) {
+        for (size_t i = 0; i < S; ++i) l[c * S + i] = cd(l[c * S + i].real() * inv_scale, l[c * S + i].imag() * inv_scale);
+        zero_pad(l.data() + c * S, pad.data(), n, m);
+        r2c_3d_T(m, m, m, pad.data(), gc.data() + c * P);
+    }
+    // (dU x W + U x dW) . gc = dU . (W x gc) + dW . (gc x U)
+    std::vector<double> gU(3 * P), gW(3 * P);
+#pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < P; ++i) {
+        const double gx = gc[i], gy = gc[P + i], gz = gc[2 * P + i];
+        const double ux = U[i], uy = U[P + i], uz = U[2 * P + i];
+        const double wx = W[i], wy = W[P + i], wz = W[2 * P + i];
+        gU[i] = wy * gz - wz * gy;
+        gU[P + i] = wz * gx - wx * gz;
+        gU[2 * P + i] = wx * gy - wy * gx;
+        gW[i] = gy * uz - gz * uy;
+        gW[P + i] = gz * ux - gx * uz;
+        gW[2 * P + i] = gx * uy - gy * ux;
+    }
+    std::vector<cd> gu(3 * S), gw(3 * S);
+    for (int c = 0; c < 3; ++c) {
+        c2r_3d_T(m, m, m, gU.data() + c * P, pad.data());
+        truncate(pad.data(), gu.data() + c * S, n, m);
+        c2r_3d_T(m, m, m, gW.data() + c * P, pad.data());
+        truncate(pad.data(), gw.data() + c * S, n, m);
+    }
+    // the pad scale, then the curl (self-adjoint) on the w branch
+    std::vector<cd> cgw(3 * S);
+    for (auto &v : gw) v = cd(v.real() * scale, v.imag() * scale);
+    curl(o, gw.data(), cgw.data());
+    for (size_t i = 0; i < 3 * S; ++i) g[i] = cd(gu[i].real() * scale, gu[i].imag() * scale) + cgw[i];
+}
+
+}  // namespace
+
+extern "C" int or_c2r_3d_T(int nx, int ny, int nz, const double *g, double *out) {
+    c2r_3d_T(nx, ny, nz, g, reinterpret_cast<cd *>(out));
+    return 0;
+}
+extern "C" int or_r2c_3d_T(int nx, int ny, int nz, const double *y, double *out) {
+    r2c_3d_T(nx, ny, nz, reinterpret_cast<const cd *>(y), out);
+    return 0;
+}
+extern "C" int or_nonlinear_vjp(int n, double l, const double *uhat, const double *lam, double *g) {
+    Ops o(n, l);
+    nonlinear_vjp(o, reinterpret_cast<const cd *>(uhat), reinterpret_cast<const cd *>(lam), reinterpret_cast<cd *>(g));
+    return 0;
+}
+
+/* VJP of one fixed-dt ETD-RK4 step (solver.rs:221-379) at the solver's
+ * current state: lam_in = (d step / d u)^T lam_out.  The state is unchanged. */
+extern "C" int or_solver_step_vjp(const or_solver *s, const double *lam_out, double *lam_in) {
+    const size_t S = s->S(), T = 3 * S;
+    const double dt = s->dt;
+    const cd *u = s->u.data();
+    const cd *lo = reinterpret_cast<const cd *>(lam_out);
+    std::vector<cd> n1(T), n2(T), n3(T), a(T), b(T), c(T);
+    // forward stages (same arithmetic as etd_rk4_step)
+    nonlinear(s->ops, u, n1.data());
+    for (int f = 0; f < 3; ++f)
+        for (size_t i = 0; i < S; ++i) {
+            const Etd &ec = s->etd[i];
+            const size_t j = f * S + i;
+            a[j] = cd(ec.eh * u[j].real() + dt * ec.a21 * n1[j].real(), ec.eh * u[j].imag() + dt * ec.a21 * n1[j].imag());
+        }
+    nonlinear(s->ops, a.data(), n2.data());
+    for (int f = 0; f < 3; ++f)
+        for (size_t i = 0; i < S; ++i) {
+            const Etd &ec = s->etd[i];
+            const size_t j = f * S + i;
+            b[j] = cd(ec.eh * u[j].real() + dt * ec.a31 * n2[j].real(), ec.eh * u[j].imag() + dt * ec.a31 * n2[j].imag());
+        }
+    nonlinear(s->ops, b.data(), n3.data());
+    for (int f = 0; f < 3; ++f)
+        for (size_t i = 0; i < S; ++i) {
+            const Etd &ec = s->etd[i];
+            const size_t j = f * S + i;
+            const cd dn(2.0 * n3[j].real() - n1[j].real(), 2.0 * n3[j].imag() - n1[j].imag());
+            c[j] = cd(ec.e * u[j].real() + dt * ec.a41 * dn.real(), ec.e * u[j].imag() + dt * ec.a41 * dn.imag());
+        }
+    // reverse sweep
+    std::vector<cd> gu(T), g1(T), g2(T), g3(T), g4(T), tmp(T);
+    for (int f = 0; f < 3; ++f)
+        for (size_t i = 0; i < S; ++i) {
+            const Etd &ec = s->etd[i];
+            const size_t j = f * S + i;
+            gu[j] = ec.e * lo[j];
+            g1[j] = dt * ec.b1 * lo[j];
+            g2[j] = dt * ec.b23 * lo[j];
+            g3[j] = dt * ec.b23 * lo[j];
+            g4[j] = dt * ec.b4 * lo[j];
+        }
+    nonlinear_vjp(s->ops, c.data(), g4.data(), tmp.data());  // d/dc
+    for (int f = 0; f < 3; ++f)
+        for (size_t i = 0; i < S; ++i) {
+            const Etd &ec = s->etd[i];
+            const size_t j = f * S + i;
+            gu[j] += ec.e * tmp[j];
+            g3[j] += 2.0 * dt * ec.a41 * tmp[j];
+            g1[j] -= dt * ec.a41 * tmp[j];
+        }
+    nonlinear_vjp(s->ops, b.data(), g3.data(), tmp.data());  // d/db
+    for (int f = 0; f < 3; ++f)
+        for (size_t i = 0; i < S; ++i) {
+            const Etd &ec = s->etd[i];
+            const size_t j = f * S + i;
+            gu[j] += ec.eh * tmp[j];
+            g2[j] += dt * ec.a31 * tmp[j];
+        }
+    nonlinear_vjp(s->ops, a.data(), g2.data(), tmp.data());  // d/da
+    for (int f = 0; f < 3; ++f)
+        for (size_t i = 0; i < S; ++i) {
+            const Etd &ec = s->etd[i];
+            const size_t j = f * S + i;
+            gu[j] += ec.eh * tmp[j];
+            g1[j] += dt * ec.a21 * tmp[j];
+        }
+    nonlinear_vjp(s->ops, u, g1.data(), tmp.data());  // d/du through N1
+    cd *li = reinterpret_cast<cd *>(lam_in);
+    for (size_t j = 0; j < T; ++j) li[j] = gu[j] + tmp[j];
+    return 0;
+}

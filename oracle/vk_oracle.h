@@ -81,6 +81,16 @@ int or_solver_diagnostics(const or_solver *s, double *out);
 int or_solver_energy_spectrum(const or_solver *s, double *out, int cap);
 int or_energy_spectrum(int n, double l, const double *uhat, double *out, int cap);
 
+/* --- discrete adjoint (no reference counterpart) ------------------------- */
+/* Transposes under the real inner product over stored entries. */
+int or_c2r_3d_T(int nx, int ny, int nz, const double *g, double *out);
+int or_r2c_3d_T(int nx, int ny, int nz, const double *y, double *out);
+/* g = (d nonlinear / d u)^T lam at uhat */
+int or_nonlinear_vjp(int n, double l, const double *uhat, const double *lam, double *g);
+/* lam_in = (d step / d u)^T lam_out at the current state (fixed dt); the
+ * state is not advanced */
+int or_solver_step_vjp(const or_solver *s, const double *lam_out, double *lam_in);
+
 /* --- initial conditions ------------------------------------------------- */
 /* Taylor-Green, physical space (vonkarman-periodic/src/ic/taylor_green.rs:10-38) */
 int or_taylor_green(int n, double l, double *u_phys);

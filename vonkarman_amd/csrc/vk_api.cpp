@@ -491,7 +491,7 @@ int vk_solver_create(int n, double l, double nu, int precision, vk_solver **out)
         }
     Scalars h0{};
     if (hipMemcpy(s->sc.p, &h0, sizeof(Scalars), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemset(s->u.p, 0, state_bytes) != hipSuccess) {
+        hipMemset(s->u.p, 0, state_bytes) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
         vk_solver_destroy(s);
         return fail(VK_ERR_HIP, "vk_solver_create: initialisation failed");
     }
