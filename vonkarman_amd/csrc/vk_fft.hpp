@@ -1,28 +1,37 @@
-// vk_fft.hpp -- LDS-tiled mixed-radix Stockham FFT core for gfx950.
+// vk_fft.hpp -- register-staged mixed-radix Stockham FFT core for gfx950.
 //
-// A workgroup transforms a tile of T independent lines of length L held in
-// LDS.  Element (position p, line c) lives at lds[p*SP + c*SC]:
-//   * column-interleaved tiles (strided x/y passes): SP = T, SC = 1, so the T
-//     lines of one position are adjacent (one 128-byte HBM row per position);
+// A workgroup transforms T independent lines of length L.  Every thread owns
+// E elements (E = T*L/NT) that stay in registers between passes; LDS is used
+// only to exchange data between passes:
+//   * before the first pass the caller fills the registers directly from HBM
+//     in the first pass's butterfly order (reg_pos_in);
+//   * pass k reads its butterfly inputs from LDS (k > 1), twiddles, runs the
+//     radix-R DFT in registers and writes the Stockham (autosort) outputs back
+//     (k < last), with a barrier between the read and the write phases;
+//   * the last pass leaves its outputs in registers at positions
+//     j + m*(L/R_last) (reg_pos_out), from where the caller stores to HBM.
+// So a P-pass plan costs P-1 LDS round trips.  Position p of line c lives at
+// lds[p*SP + c*SC] during the exchanges:
+//   * column-interleaved tiles (strided x/y passes): SP = T, SC = 1, the T
+//     lines of one position are adjacent (one HBM row per position);
 //   * line-major tiles (contiguous z passes): SP = 1, SC = line pitch.
-// Each pass reads its butterfly inputs from LDS into registers, barriers, then
-// twiddles, runs the radix-R DFT in registers and writes the Stockham
-// (autosort) output positions back into the same LDS buffer.  Output is in
-// natural order.  Twiddles come from a per-length table W_L[q] = e^{-2 pi i q/L}
-// generated on the host in extended precision and rounded once.
+// Because the last pass of a plan leaves element e at the position where the
+// first pass of the reversed plan expects element e, an inverse transform can
+// feed a forward transform register-to-register (pointwise work in between
+// needs no reordering).
 //
 // Direction D = -1 is the forward transform (e^{-2 pi i jk/L}); D = +1 is the
-// unnormalised inverse.  Everything here is __host__ __device__ where it can
-// be, so the index arithmetic is exercised by a host build in the tests.
+// unnormalised inverse.  Twiddles come from a per-length table
+// W_L[q] = e^{-2 pi i q/L} generated on the host in extended precision.  The
+// index arithmetic and the radix kernels are __host__ __device__ so a host
+// build executes them in the tests.
 #pragma once
 
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 #define VK_HD __host__ __device__ __forceinline__
-#define VK_DEV __device__ __forceinline__
 #else
 #define VK_HD inline
-#define VK_DEV inline
 #endif
 
 namespace vk {
@@ -48,6 +57,66 @@ template <int D, typename R>
 VK_HD cpx<R> cmul_i(cpx<R> a) {
     if constexpr (D > 0) return {-a.im, a.re};
     else return {a.im, -a.re};
+}
+
+// cos and sin of 2 pi q / 48, q = 0..47 (extended precision, rounded once)
+static constexpr double kTab48[48][2] = {
+    {1, 0},
+    {0.99144486137381038, 0.1305261922200516},
+    {0.96592582628906831, 0.25881904510252074},
+    {0.92387953251128674, 0.38268343236508978},
+    {0.8660254037844386, 0.5},
+    {0.79335334029123517, 0.60876142900872066},
+    {0.70710678118654757, 0.70710678118654757},
+    {0.60876142900872066, 0.79335334029123517},
+    {0.5, 0.8660254037844386},
+    {0.38268343236508978, 0.92387953251128674},
+    {0.25881904510252074, 0.96592582628906831},
+    {0.1305261922200516, 0.99144486137381038},
+    {-2.5082788063341661e-20, 1},
+    {-0.1305261922200516, 0.99144486137381038},
+    {-0.25881904510252074, 0.96592582628906831},
+    {-0.38268343236508978, 0.92387953251128674},
+    {-0.5, 0.8660254037844386},
+    {-0.60876142900872066, 0.79335334029123517},
+    {-0.70710678118654757, 0.70710678118654757},
+    {-0.79335334029123517, 0.60876142900872066},
+    {-0.8660254037844386, 0.5},
+    {-0.92387953251128674, 0.38268343236508978},
+    {-0.96592582628906831, 0.25881904510252074},
+    {-0.99144486137381038, 0.1305261922200516},
+    {-1, -5.0165576126683323e-20},
+    {-0.99144486137381038, -0.1305261922200516},
+    {-0.96592582628906831, -0.25881904510252074},
+    {-0.92387953251128674, -0.38268343236508978},
+    {-0.8660254037844386, -0.5},
+    {-0.79335334029123517, -0.60876142900872066},
+    {-0.70710678118654757, -0.70710678118654757},
+    {-0.60876142900872066, -0.79335334029123517},
+    {-0.5, -0.8660254037844386},
+    {-0.38268343236508978, -0.92387953251128674},
+    {-0.25881904510252074, -0.96592582628906831},
+    {-0.1305261922200516, -0.99144486137381038},
+    {1.8366858143857543e-19, -1},
+    {0.1305261922200516, -0.99144486137381038},
+    {0.25881904510252074, -0.96592582628906831},
+    {0.38268343236508978, -0.92387953251128674},
+    {0.5, -0.8660254037844386},
+    {0.60876142900872066, -0.79335334029123517},
+    {0.70710678118654757, -0.70710678118654757},
+    {0.79335334029123517, -0.60876142900872066},
+    {0.8660254037844386, -0.5},
+    {0.92387953251128674, -0.38268343236508978},
+    {0.96592582628906831, -0.25881904510252074},
+    {0.99144486137381038, -0.1305261922200516},
+};
+
+// e^{D 2 pi i t / RR} for RR dividing 48
+template <int RR, int D, typename R>
+VK_HD cpx<R> twc(int t) {
+    static_assert(48 % RR == 0, "constant twiddles cover divisors of 48");
+    const int q = ((t % RR) * (48 / RR)) % 48;
+    return {R(kTab48[q][0]), R(D) * R(kTab48[q][1])};
 }
 
 // ---------------------------------------------------------------------------
@@ -103,11 +172,38 @@ VK_HD void dft8(cpx<R> *v) {
 }
 
 template <int RAD, int D, typename R>
+VK_HD void dft(cpx<R> *v);
+
+// composite RR = A*B by Cooley-Tukey with constant twiddles:
+//   y[a][k2] = sum_n2 x[a + A n2] w_B^{n2 k2};  y *= w_RR^{a k2};
+//   X[k2 + B k1] = sum_a y[a][k2] w_A^{a k1}
+template <int A, int B, int D, typename R>
+VK_HD void dft_ct(cpx<R> *v) {
+    constexpr int RR = A * B;
+    cpx<R> y[RR];
+    for (int a = 0; a < A; ++a) {
+        cpx<R> t[B];
+        for (int n2 = 0; n2 < B; ++n2) t[n2] = v[a + A * n2];
+        dft<B, D>(t);
+        for (int k2 = 0; k2 < B; ++k2) y[a * B + k2] = (a * k2) % RR ? cmul(t[k2], twc<RR, D, R>(a * k2)) : t[k2];
+    }
+    for (int k2 = 0; k2 < B; ++k2) {
+        cpx<R> t[A];
+        for (int a = 0; a < A; ++a) t[a] = y[a * B + k2];
+        dft<A, D>(t);
+        for (int k1 = 0; k1 < A; ++k1) v[k2 + B * k1] = t[k1];
+    }
+}
+
+template <int RAD, int D, typename R>
 VK_HD void dft(cpx<R> *v) {
     if constexpr (RAD == 2) dft2<D>(v);
     else if constexpr (RAD == 3) dft3<D>(v);
     else if constexpr (RAD == 4) dft4<D>(v);
+    else if constexpr (RAD == 6) dft_ct<2, 3, D>(v);
     else if constexpr (RAD == 8) dft8<D>(v);
+    else if constexpr (RAD == 12) dft_ct<4, 3, D>(v);
+    else if constexpr (RAD == 16) dft_ct<4, 4, D>(v);
     else static_assert(RAD == 2, "unsupported radix");
 }
 
@@ -135,92 +231,213 @@ VK_HD int bfly_out(int j, int m) { return (j / NS) * NS * RAD + (j % NS) + m * N
 template <int L, int RAD, int NS>
 VK_HD int bfly_tw(int j, int m) { return m * (j % NS) * (L / (NS * RAD)); }
 
-template <int... R>
-struct RadixList {};
-
-#if defined(__HIPCC__)
-template <int L, int RAD, int NS, int D, int T, int NT, int SP, int SC, typename R>
-__device__ __forceinline__ void stockham_pass(cpx<R> *lds, const cpx<R> *__restrict__ tw, int tid) {
-    constexpr int NB = T * (L / RAD);
-    constexpr int BPT = (NB + NT - 1) / NT;
-    cpx<R> v[BPT][RAD];
-#pragma unroll
-    for (int b = 0; b < BPT; ++b) {
-        const int g = tid + b * NT;
-        if (NB % NT == 0 || g < NB) {
-            int c, j;
-            bfly_coords<L, RAD, T, SC>(g, c, j);
-#pragma unroll
-            for (int m = 0; m < RAD; ++m) v[b][m] = lds[bfly_in<L, RAD>(j, m) * SP + c * SC];
+// Twiddle the inputs m = 1..RAD-1 of butterfly j: multiply by w^m (D < 0) or
+// conj(w)^m (D > 0), w = W_L^{(j % NS) L/(NS RAD)}.  Only w is read from the
+// table; its powers are formed by repeated multiplication (a few ulps).
+template <int L, int RAD, int NS, int D, typename R>
+VK_HD void twiddle_bfly(cpx<R> *v, int j, const cpx<R> *tw) {
+    if constexpr (NS > 1) {
+        const cpx<R> w1 = tw[bfly_tw<L, RAD, NS>(j, 1)];
+        cpx<R> w = w1;
+        for (int m = 1; m < RAD; ++m) {
+            v[m] = D < 0 ? cmul(v[m], w) : cmulc(v[m], w);
+            if (m + 1 < RAD) w = cmul(w, w1);
         }
     }
-    __syncthreads();
-#pragma unroll
-    for (int b = 0; b < BPT; ++b) {
-        const int g = tid + b * NT;
-        if (NB % NT == 0 || g < NB) {
-            int c, j;
-            bfly_coords<L, RAD, T, SC>(g, c, j);
-            if constexpr (NS > 1) {
-#pragma unroll
-                for (int m = 1; m < RAD; ++m) {
-                    const cpx<R> w = tw[bfly_tw<L, RAD, NS>(j, m)];
-                    v[b][m] = D < 0 ? cmul(v[b][m], w) : cmulc(v[b][m], w);
-                }
-            }
-            dft<RAD, D>(v[b]);
-#pragma unroll
-            for (int m = 0; m < RAD; ++m) lds[bfly_out<L, RAD, NS>(j, m) * SP + c * SC] = v[b][m];
-        }
+}
+
+// radix lists
+template <int... RS>
+struct RadixList {
+    static constexpr int n = sizeof...(RS);
+    static constexpr int r[n] = {RS...};
+    static constexpr int first = r[0];
+    static constexpr int last = r[n - 1];
+    static constexpr int product() {
+        int p = 1;
+        for (int i = 0; i < n; ++i) p *= r[i];
+        return p;
     }
-    __syncthreads();
+};
+
+template <typename A, typename B>
+struct Concat;
+template <int... A, int... B>
+struct Concat<RadixList<A...>, RadixList<B...>> {
+    using type = RadixList<A..., B...>;
+};
+template <typename L>
+struct Reverse;
+template <int R0>
+struct Reverse<RadixList<R0>> {
+    using type = RadixList<R0>;
+};
+template <int R0, int R1, int... RS>
+struct Reverse<RadixList<R0, R1, RS...>> {
+    using type = typename Concat<typename Reverse<RadixList<R1, RS...>>::type, RadixList<R0>>::type;
+};
+
+// element e of a thread's registers before the transform (first radix R1):
+// line c and position p
+template <int L, int R1, int T, int NT, int SC>
+VK_HD void reg_pos_in(int tid, int e, int &c, int &p) {
+    int j;
+    bfly_coords<L, R1, T, SC>(tid + (e / R1) * NT, c, j);
+    p = bfly_in<L, R1>(j, e % R1);
+}
+// after the transform (last radix RP): position j + m*(L/RP)
+template <int L, int RP, int T, int NT, int SC>
+VK_HD void reg_pos_out(int tid, int e, int &c, int &p) {
+    int j;
+    bfly_coords<L, RP, T, SC>(tid + (e / RP) * NT, c, j);
+    p = j + (e % RP) * (L / RP);
 }
 
-template <int L, int NS, int D, int T, int NT, int SP, int SC, typename R, int RAD, int... REST>
-__device__ __forceinline__ void stockham_run(cpx<R> *lds, const cpx<R> *__restrict__ tw, int tid) {
-    stockham_pass<L, RAD, NS, D, T, NT, SP, SC>(lds, tw, tid);
-    if constexpr (sizeof...(REST) > 0) stockham_run<L, NS * RAD, D, T, NT, SP, SC, R, REST...>(lds, tw, tid);
+// Column-interleaved tiles with NT a multiple of T: every element of a thread
+// lies in column tid % T; only the position depends on the element.
+template <int L, int R1, int T, int NT>
+VK_HD int ci_pos_in(int tid, int e) {
+    static_assert(NT % T == 0, "threads must cover whole rows");
+    return bfly_in<L, R1>(tid / T + (e / R1) * (NT / T), e % R1);
+}
+template <int L, int RP, int T, int NT>
+VK_HD int ci_pos_out(int tid, int e) {
+    static_assert(NT % T == 0, "threads must cover whole rows");
+    return tid / T + (e / RP) * (NT / T) + (e % RP) * (L / RP);
 }
 
-template <int L, int D, int T, int NT, int SP, int SC, typename R, int... RS>
-__device__ __forceinline__ void stockham_list(RadixList<RS...>, cpx<R> *lds, const cpx<R> *__restrict__ tw,
-                                              int tid) {
-    stockham_run<L, 1, D, T, NT, SP, SC, R, RS...>(lds, tw, tid);
-}
-#endif
-
-// radix plans per length (radices applied in order, NS growing 1, R1, R1 R2...)
+// ---------------------------------------------------------------------------
+// plans.  Each kernel class picks radices (first to last) and E, the number of
+// elements per thread (divisible by every radix).
+// ---------------------------------------------------------------------------
+// Strided passes: fewest LDS round trips with E <= 24.
 template <int L>
-struct Plan;
+struct PlanS;
+// Passes that keep three fields of outputs in registers (E = 12, last radix a
+// multiple of 3 for the 3/2-padded lengths so every butterfly keeps the same
+// 2/3 of its outputs after truncation).
+template <int L>
+struct PlanE12;
+// Power-of-two lengths with E = 8 (low register use).
+template <int L>
+struct PlanE8;
+// The smaller-E plan of a length: PlanE12 for 3/2-padded lengths, PlanE8 else.
 
-#define VK_PLAN(LEN, ...)                          \
+#define VK_PLAN(NAME, LEN, EPT, ...)               \
     template <>                                    \
-    struct Plan<LEN> {                             \
-        using radix_list = RadixList<__VA_ARGS__>; \
+    struct NAME<LEN> {                             \
+        using radices = RadixList<__VA_ARGS__>;    \
+        static constexpr int E = EPT;              \
     };
 
-VK_PLAN(8, 8)
-VK_PLAN(12, 4, 3)
-VK_PLAN(16, 4, 4)
-VK_PLAN(24, 8, 3)
-VK_PLAN(32, 8, 4)
-VK_PLAN(48, 4, 4, 3)
-VK_PLAN(64, 8, 8)
-VK_PLAN(96, 8, 4, 3)
-VK_PLAN(128, 8, 4, 4)
-VK_PLAN(192, 8, 8, 3)
-VK_PLAN(256, 8, 8, 4)
-VK_PLAN(384, 8, 4, 4, 3)
-VK_PLAN(512, 8, 8, 8)
-VK_PLAN(768, 8, 8, 4, 3)
+VK_PLAN(PlanS, 8, 8, 8)
+VK_PLAN(PlanS, 12, 12, 12)
+VK_PLAN(PlanS, 16, 16, 16)
+VK_PLAN(PlanS, 24, 24, 8, 3)
+VK_PLAN(PlanS, 32, 16, 8, 4)
+VK_PLAN(PlanS, 48, 24, 8, 6)
+VK_PLAN(PlanS, 64, 16, 8, 8)
+VK_PLAN(PlanS, 96, 24, 8, 12)
+VK_PLAN(PlanS, 128, 16, 16, 8)
+VK_PLAN(PlanS, 192, 24, 8, 8, 3)
+VK_PLAN(PlanS, 256, 16, 16, 16)
+VK_PLAN(PlanS, 384, 24, 8, 8, 6)
+VK_PLAN(PlanS, 512, 16, 8, 8, 8)
+VK_PLAN(PlanS, 768, 24, 8, 8, 12)
+
+VK_PLAN(PlanE8, 8, 8, 8)
+VK_PLAN(PlanE8, 16, 8, 8, 2)
+VK_PLAN(PlanE8, 32, 8, 8, 4)
+VK_PLAN(PlanE8, 64, 8, 8, 8)
+VK_PLAN(PlanE8, 128, 8, 8, 4, 4)
+VK_PLAN(PlanE8, 256, 8, 8, 8, 4)
+VK_PLAN(PlanE8, 512, 8, 8, 8, 8)
+
+VK_PLAN(PlanE12, 12, 12, 12)
+VK_PLAN(PlanE12, 24, 12, 4, 6)
+VK_PLAN(PlanE12, 48, 12, 4, 12)
+VK_PLAN(PlanE12, 96, 12, 4, 4, 6)
+VK_PLAN(PlanE12, 192, 12, 4, 4, 12)
+VK_PLAN(PlanE12, 384, 12, 4, 4, 4, 6)
+VK_PLAN(PlanE12, 768, 12, 4, 4, 4, 12)
 #undef VK_PLAN
 
+template <int L, bool PADDED = (L % 3 == 0)>
+struct PlanSmall {
+    using type = PlanE12<L>;
+};
+template <int L>
+struct PlanSmall<L, false> {
+    using type = PlanE8<L>;
+};
+
+// launch geometry of a plan: T lines per block, at least ROW_BYTES of each
+// position adjacent (strided tiles), NT = T*L/E threads (>= 64)
+template <int L, int E, typename R, int ROW_BYTES = 128>
+struct Geom {
+    static constexpr int T0 = ROW_BYTES / (2 * (int)sizeof(R));
+    static constexpr int TMIN = (64 * E + L - 1) / L;
+    static constexpr int T = T0 >= TMIN ? T0 : ((TMIN + T0 - 1) / T0) * T0;
+    static constexpr int NT = T * L / E;
+    static_assert((T * L) % E == 0, "tile must split evenly");
+};
+
 #if defined(__HIPCC__)
-// Transform the tile in place: entry and exit are barrier-separated, data in
-// natural order at lds[p*SP + c*SC].
-template <int L, int D, int T, int NT, int SP, int SC, typename R>
-__device__ __forceinline__ void fft_tile(cpx<R> *lds, const cpx<R> *__restrict__ tw, int tid) {
-    stockham_list<L, D, T, NT, SP, SC>(typename Plan<L>::radix_list{}, lds, tw, tid);
+// one pass of the register-staged transform
+template <int L, int NS, int D, int T, int NT, int SP, int SC, int E, bool FIRST, typename R, int RAD, int... REST>
+__device__ __forceinline__ void fft_reg_run(cpx<R> (&v)[E], cpx<R> *lds, const cpx<R> *__restrict__ tw, int tid) {
+    static_assert(E % RAD == 0, "E must be a multiple of every radix");
+    constexpr int BPT = E / RAD;
+    static_assert(T * (L / RAD) == BPT * NT, "butterflies must split evenly over the threads");
+    if constexpr (!FIRST) {
+#pragma unroll
+        for (int b = 0; b < BPT; ++b) {
+            int c, j;
+            bfly_coords<L, RAD, T, SC>(tid + b * NT, c, j);
+#pragma unroll
+            for (int m = 0; m < RAD; ++m) v[b * RAD + m] = lds[bfly_in<L, RAD>(j, m) * SP + c * SC];
+        }
+        if constexpr (sizeof...(REST) > 0) __syncthreads();
+    }
+#pragma unroll
+    for (int b = 0; b < BPT; ++b) {
+        int c, j;
+        bfly_coords<L, RAD, T, SC>(tid + b * NT, c, j);
+        twiddle_bfly<L, RAD, NS, D>(&v[b * RAD], j, tw);
+        dft<RAD, D>(&v[b * RAD]);
+    }
+    if constexpr (sizeof...(REST) > 0) {
+#pragma unroll
+        for (int b = 0; b < BPT; ++b) {
+            int c, j;
+            bfly_coords<L, RAD, T, SC>(tid + b * NT, c, j);
+#pragma unroll
+            for (int m = 0; m < RAD; ++m) lds[bfly_out<L, RAD, NS>(j, m) * SP + c * SC] = v[b * RAD + m];
+        }
+        __syncthreads();
+        fft_reg_run<L, NS * RAD, D, T, NT, SP, SC, E, false, R, REST...>(v, lds, tw, tid);
+    }
+}
+
+template <int L, int D, int T, int NT, int SP, int SC, int E, typename R, int... RS>
+__device__ __forceinline__ void fft_reg_list(RadixList<RS...>, cpx<R> (&v)[E], cpx<R> *lds,
+                                             const cpx<R> *__restrict__ tw, int tid) {
+    fft_reg_run<L, 1, D, T, NT, SP, SC, E, true, R, RS...>(v, lds, tw, tid);
+}
+
+// Transform the registers v (filled in reg_pos_in order of the plan's first
+// radix) in place; on return they hold reg_pos_out positions of its last
+// radix.  The caller must __syncthreads() before it writes LDS again.
+template <int L, int D, int T, int NT, int SP, int SC, typename PL, int E, typename R>
+__device__ __forceinline__ void fft_reg(cpx<R> (&v)[E], cpx<R> *lds, const cpx<R> *__restrict__ tw, int tid) {
+    static_assert(PL::product() == L, "plan must multiply to L");
+    // An opaque copy of the table pointer: the twiddles of a transform are
+    // loaded where they are used instead of being hoisted (with their powers)
+    // out of the callers' per-field loops, which would hold them all in
+    // registers at once.
+    const cpx<R> *twp = tw;
+    asm volatile("" : "+s"(twp));
+    fft_reg_list<L, D, T, NT, SP, SC>(PL{}, v, lds, twp, tid);
 }
 #endif
 
@@ -230,23 +447,9 @@ __device__ __forceinline__ void fft_tile(cpx<R> *lds, const cpx<R> *__restrict__
 // ---------------------------------------------------------------------------
 // unpadded index i in [0, n) -> FFT position in [0, L)
 VK_HD int pad_pos(int i, int n, int L) { return i < n / 2 ? i : L - (n - i); }
+// FFT position p in [0, L) -> unpadded index, or -1 in the zero gap
+VK_HD int unpad_index(int p, int n, int L) { return p < n / 2 ? p : (p >= L - n / 2 ? p - (L - n) : -1); }
 // signed wavenumber of unpadded index i (spectral_ops.rs:55-61)
 VK_HD int wavenum(int i, int n) { return i < n / 2 ? i : i - n; }
-
-// tile geometry per length and real type
-template <int L, typename R>
-struct TileCfg {
-    // columns per strided tile: one 128-byte row per position (64 bytes at
-    // L=768 so the tile stays at 48 KiB of LDS)
-    static constexpr int ROW_BYTES = L >= 768 ? 64 : 128;
-    static constexpr int T = ROW_BYTES / (2 * (int)sizeof(R));
-    static constexpr int NT_raw = L * T / 2;
-    static constexpr int NT = NT_raw >= 256 ? 256 : (NT_raw <= 64 ? 64 : NT_raw);
-    // lines per contiguous (z) tile: even, about 2048 complex elements
-    static constexpr int TZ_raw = 2048 / L;
-    static constexpr int TZ = TZ_raw < 2 ? 2 : (TZ_raw > 16 ? 16 : (TZ_raw & ~1));
-    static constexpr int NTZ_raw = TZ * L / 4;
-    static constexpr int NTZ = NTZ_raw >= 256 ? 256 : (NTZ_raw <= 64 ? 64 : NTZ_raw);
-};
 
 }  // namespace vk

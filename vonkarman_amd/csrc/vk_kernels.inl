@@ -21,6 +21,15 @@ __device__ __forceinline__ cpx<TO> cvt(cpx<TI> a) {
     return {TO(a.re), TO(a.im)};
 }
 
+// Make a pointer opaque to the optimiser (no instruction): address arithmetic
+// derived from it stays inside the loop iteration that uses it instead of
+// being hoisted, with its registers, out of a per-field loop.
+template <typename P>
+__device__ __forceinline__ P *opaque(P *p) {
+    asm volatile("" : "+s"(p));
+    return p;
+}
+
 __device__ __forceinline__ long long col_addr(int col, int inner, long long os) {
     return (long long)(col / inner) * os + (col % inner);
 }
@@ -29,113 +38,113 @@ __device__ __forceinline__ long long col_addr(int col, int inner, long long os) 
 // Inverse strided pass: lines of LIN spectral inputs zero-padded to L, inverse
 // DFT, all L physical outputs stored.  Curl terms are formed on the way in:
 // i*k multiplications commute with transforms along the other axes, so the x
-// pass forms i kx (.) and the y pass forms i ky (.) and i kz (.).
+// pass forms i kx (.) and the y pass forms i ky (.) and i kz (.).  Each output
+// field re-reads its input fields (repeat reads of a tile hit the L2/MALL).
 // ===========================================================================
+// single-field modes use the low-pass-count plan; modes that combine fields
+// use the smaller-E plan to stay within the register budget
+template <int L, int MODE>
+struct InvPlan {
+    using type = typename PlanSmall<L>::type;
+};
+template <int L>
+struct InvPlan<L, INV_PLAIN> {
+    using type = PlanS<L>;
+};
+template <int L>
+struct InvPlan<L, INV_U3> {
+    using type = PlanS<L>;
+};
+template <int L, int MODE, typename TW>
+using InvGeom = Geom<L, InvPlan<L, MODE>::type::E, TW>;
+
 template <int L, int LIN, int MODE, typename TI, typename TW>
-__global__ __launch_bounds__((TileCfg<L, TW>::NT)) void k_inv(const cpx<TI> *__restrict__ in,
-                                                            cpx<TW> *__restrict__ out,
-                                                            const cpx<TW> *__restrict__ tw, Strided g,
-                                                            int n, double ks, TW scale) {
-    using C = TileCfg<L, TW>;
-    constexpr int T = C::T, NT = C::NT;
-    constexpr int NE = LIN * T;
-    constexpr int EPT = (NE + NT - 1) / NT;
+__global__ __launch_bounds__((InvGeom<L, MODE, TW>::NT)) void k_inv(const cpx<TI> *__restrict__ in,
+                                                              cpx<TW> *__restrict__ out,
+                                                              const cpx<TW> *__restrict__ tw, Strided g, int n,
+                                                              double ks, TW scale) {
+    using P = typename InvPlan<L, MODE>::type;
+    using PL = typename P::radices;
+    constexpr int E = P::E;
+    constexpr int T = InvGeom<L, MODE, TW>::T, NT = InvGeom<L, MODE, TW>::NT;
     __shared__ cpx<TW> lds[L * T];
     const int tid = threadIdx.x;
     const int col0 = blockIdx.x * T;
     const cpx<TW> zero = {TW(0), TW(0)};
+    cpx<TW> v[E];
 
-    cpx<TW> ra[EPT], rb[EPT];
-
-    auto load = [&](cpx<TW> *r, int f) {
-#pragma unroll
-        for (int i = 0; i < EPT; ++i) {
-            const int e = tid + i * NT;
-            r[i] = zero;
-            if (NE % NT == 0 || e < NE) {
-                const int p = e / T, col = col0 + e % T;
-                if (col < g.ncols && (col % g.hp) < g.hvalid)
-                    r[i] = cvt<TW>(in[f * g.in_fs + col_addr(col, g.in_inner, g.in_os) + p * g.in_ls]);
-            }
-        }
+    // this thread's column (NT is a multiple of T) and its base offsets
+    const int col = col0 + tid % T;
+    const bool colok = col < g.ncols && (col % g.hp) < g.hvalid;
+    const long long cin = colok ? col_addr(col, g.in_inner, g.in_os) : 0;
+    const long long cout = colok ? col_addr(col, g.out_inner, g.out_os) : 0;
+    const TW kz_col = TW(double(col % g.hp) * ks);  // kz of the column (y pass)
+    // element e -> unpadded input index (-1 in the zero gap)
+    auto uidx = [&](int e) -> int {
+        const int p = ci_pos_in<L, PL::first, T, NT>(tid, e);
+        return LIN == L ? p : unpad_index(p, LIN, L);
     };
-    // wavenumber along the line for element i (unpadded index -> signed k)
-    auto kline = [&](int i) -> TW {
-        const int p = (tid + i * NT) / T;
-        return TW(double(wavenum(p, LIN == L ? n : LIN)) * ks);
+    const cpx<TI> *src = in;
+    auto load = [&](int f, int e) -> cpx<TW> {
+        const int i = uidx(e);
+        if (!colok || i < 0) return zero;
+        return cvt<TW>(src[f * g.in_fs + cin + (long long)i * g.in_ls]);
     };
-    auto kcol = [&](int i) -> TW {  // kz from the column (y pass only)
-        const int col = col0 + (tid + i * NT) % T;
-        return TW(double(col % g.hp) * ks);
+    auto kline = [&](int e) -> TW {
+        const int i = uidx(e);
+        return TW(double(wavenum(i < 0 ? 0 : i, LIN == L ? n : LIN)) * ks);
     };
-    auto emit = [&](int fo, auto value) {
-        __syncthreads();
-        if constexpr (LIN < L) {
-            for (int e = tid; e < (L - LIN) * T; e += NT) lds[(LIN / 2 + e / T) * T + e % T] = zero;
-        }
-#pragma unroll
-        for (int i = 0; i < EPT; ++i) {
-            const int e = tid + i * NT;
-            if (NE % NT == 0 || e < NE) {
-                const int p = e / T, c = e % T;
-                lds[pad_pos(p, LIN, L) * T + c] = value(i);
-            }
-        }
-        __syncthreads();
-        fft_tile<L, +1, T, NT, T, 1>(lds, tw, tid);
-        for (int e = tid; e < L * T; e += NT) {
-            const int p = e / T, col = col0 + e % T;
-            if (col < g.ncols && (col % g.hp) < g.hvalid)
-                out[fo * g.out_fs + col_addr(col, g.out_inner, g.out_os) + p * g.out_ls] =
-                    cscale(lds[p * T + e % T], scale);
-        }
-    };
+    auto kcol = [&](int) -> TW { return kz_col; };
     auto ik = [](TW k, cpx<TW> a) -> cpx<TW> { return {-k * a.im, k * a.re}; };
-
-    if constexpr (MODE == INV_PLAIN) {
-        load(ra, 0);
-        emit(0, [&](int i) { return ra[i]; });
-    } else if constexpr (MODE == INV_U3) {
-        for (int f = 0; f < 3; ++f) {
-            load(ra, f);
-            emit(f, [&](int i) { return ra[i]; });
+    // outputs by mode:
+    //   PLAIN: 0 <- in 0;  U3: f <- in f
+    //   U3DX (x pass): 0 ux, 1 uy, 2 uz, 3 i kx uz, 4 i kx uy
+    //   UOMEGA (y pass, inputs 0 ux, 1 uy, 2 uz, 3 ax = i kx uz, 4 bx = i kx uy):
+    //     0..2 u, 3 wx = i (ky uz - kz uy), 4 wy = i kz ux - ax, 5 wz = bx - i ky ux
+    //   OMEGA: the three w outputs only
+    constexpr int NO = MODE == INV_PLAIN ? 1 : MODE == INV_U3 ? 3 : MODE == INV_U3DX ? 5 : MODE == INV_OMEGA ? 3 : 6;
+#pragma unroll 1
+    for (int fo = 0; fo < NO; ++fo) {
+        src = opaque(in);
+        // kind of output fo: 0..2 plain copy of input fo; 3 i kx uz; 4 i kx uy; 5 wx; 6 wy; 7 wz
+        int kind = fo;
+        if constexpr (MODE == INV_OMEGA) kind = 5 + fo;
+        if constexpr (MODE == INV_UOMEGA) kind = fo < 3 ? fo : 2 + fo;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            switch (kind) {
+                case 3: v[e] = ik(kline(e), load(2, e)); break;
+                case 4: v[e] = ik(kline(e), load(1, e)); break;
+                case 5: {
+                    const TW ky = kline(e), kz = kcol(e);
+                    const cpx<TW> uy = load(1, e), uz = load(2, e);
+                    v[e] = ik(TW(1), cpx<TW>{ky * uz.re - kz * uy.re, ky * uz.im - kz * uy.im});
+                    break;
+                }
+                case 6: {
+                    const TW kz = kcol(e);
+                    const cpx<TW> ux = load(0, e), ax = load(3, e);
+                    v[e] = {-kz * ux.im - ax.re, kz * ux.re - ax.im};
+                    break;
+                }
+                case 7: {
+                    const TW ky = kline(e);
+                    const cpx<TW> ux = load(0, e), bx = load(4, e);
+                    v[e] = {bx.re + ky * ux.im, bx.im - ky * ux.re};
+                    break;
+                }
+                default: v[e] = load(kind, e); break;
+            }
         }
-    } else if constexpr (MODE == INV_U3DX) {
-        // outputs: 0 ux, 1 uy, 2 uz, 3 i kx uz, 4 i kx uy
-        load(ra, 0);
-        emit(0, [&](int i) { return ra[i]; });
-        load(ra, 1);
-        emit(1, [&](int i) { return ra[i]; });
-        emit(4, [&](int i) { return ik(kline(i), ra[i]); });
-        load(ra, 2);
-        emit(2, [&](int i) { return ra[i]; });
-        emit(3, [&](int i) { return ik(kline(i), ra[i]); });
-    } else {
-        // y pass; inputs: 0 ux, 1 uy, 2 uz, 3 ax = i kx uz, 4 bx = i kx uy
-        //   wx = i (ky uz - kz uy),  wy = i kz ux - ax,  wz = bx - i ky ux
-        constexpr int OU = (MODE == INV_UOMEGA) ? 0 : -1;  // u outputs 0..2 when present
-        constexpr int OW = (MODE == INV_UOMEGA) ? 3 : 0;   // first w output
-        load(ra, 0);
-        if constexpr (OU >= 0) emit(0, [&](int i) { return ra[i]; });
-        load(rb, 4);
-        emit(OW + 2, [&](int i) {
-            const TW ky = kline(i);
-            return cpx<TW>{rb[i].re + ky * ra[i].im, rb[i].im - ky * ra[i].re};
-        });
-        load(rb, 3);
-        emit(OW + 1, [&](int i) {
-            const TW kz = kcol(i);
-            return cpx<TW>{-kz * ra[i].im - rb[i].re, kz * ra[i].re - rb[i].im};
-        });
-        load(ra, 1);
-        if constexpr (OU >= 0) emit(1, [&](int i) { return ra[i]; });
-        load(rb, 2);
-        if constexpr (OU >= 0) emit(2, [&](int i) { return rb[i]; });
-        emit(OW + 0, [&](int i) {
-            const TW ky = kline(i), kz = kcol(i);
-            const cpx<TW> t = {ky * rb[i].re - kz * ra[i].re, ky * rb[i].im - kz * ra[i].im};
-            return ik(TW(1), t);
-        });
+        fft_reg<L, +1, T, NT, T, 1, PL>(v, lds, tw, tid);
+        if (colok) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int p = ci_pos_out<L, PL::last, T, NT>(tid, e);
+                out[fo * g.out_fs + cout + (long long)p * g.out_ls] = cscale(v[e], scale);
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -190,72 +199,107 @@ __device__ __forceinline__ void stage_epilogue(const StageArgs &sa, long long id
     }
 }
 
+template <int L, int MODE>
+struct FwdPlan {
+    using type = PlanS<L>;
+};
+template <int L>
+struct FwdPlan<L, FWD_STAGE> {
+    using type = PlanE12<L>;
+};
+template <int L, int MODE, typename TW>
+using FwdGeom = Geom<L, FwdPlan<L, MODE>::type::E, TW>;
+
 template <int L, int LOUT, int MODE, int NF, typename TW>
-__global__ __launch_bounds__((TileCfg<L, TW>::NT)) void k_fwd(const cpx<TW> *__restrict__ in,
-                                                            cpx<TW> *__restrict__ out,
-                                                            const cpx<TW> *__restrict__ tw, Strided g,
-                                                            int n, double ks, TW scale, StageArgs sa) {
-    using C = TileCfg<L, TW>;
-    constexpr int T = C::T, NT = C::NT;
-    constexpr int NE = LOUT * T;
-    constexpr int EPT = (NE + NT - 1) / NT;
+__global__ __launch_bounds__((FwdGeom<L, MODE, TW>::NT)) void k_fwd(const cpx<TW> *__restrict__ in,
+                                                                    cpx<TW> *__restrict__ out,
+                                                                    const cpx<TW> *__restrict__ tw, Strided g,
+                                                                    int n, double ks, TW scale, StageArgs sa) {
+    using P = typename FwdPlan<L, MODE>::type;
+    using PL = typename P::radices;
+    constexpr int E = P::E;
+    constexpr int T = FwdGeom<L, MODE, TW>::T, NT = FwdGeom<L, MODE, TW>::NT;
+    constexpr int RP = PL::last;
     __shared__ cpx<TW> lds[L * T];
     const int tid = threadIdx.x;
     const int col0 = blockIdx.x * T;
     const cpx<TW> zero = {TW(0), TW(0)};
-    cpx<TW> r[MODE == FWD_STAGE ? 3 : 1][EPT];
+    cpx<TW> v[E];
+    // kept outputs of the last radix under truncation: m in [0, RP/3) u [2RP/3, RP)
+    constexpr int KR = (MODE == FWD_STAGE) ? 2 * RP / 3 : 1;
+    constexpr int K = (MODE == FWD_STAGE) ? (E / RP) * KR : 1;
+    static_assert(MODE != FWD_STAGE || (RP % 3 == 0 && 3 * LOUT == 2 * L), "stage pass needs a 3/2-padded plan");
+    cpx<TW> r[MODE == FWD_STAGE ? 3 : 1][K];
 
-    for (int f = 0; f < NF; ++f) {
-        __syncthreads();
-        for (int e = tid; e < L * T; e += NT) {
-            const int p = e / T, col = col0 + e % T;
-            cpx<TW> v = zero;
-            if (col < g.ncols && (col % g.hp) < g.hvalid)
-                v = in[f * g.in_fs + col_addr(col, g.in_inner, g.in_os) + p * g.in_ls];
-            lds[p * T + e % T] = v;
-        }
-        __syncthreads();
-        fft_tile<L, -1, T, NT, T, 1>(lds, tw, tid);
+    const int col = col0 + tid % T;
+    const bool colok = col < g.ncols && (col % g.hp) < g.hvalid;
+    const long long cin = colok ? col_addr(col, g.in_inner, g.in_os) : 0;
+    const long long cout = colok ? col_addr(col, g.out_inner, g.out_os) : 0;
+    auto load_field = [&](int f) {
+        const cpx<TW> *src = opaque(in) + f * g.in_fs + cin;
 #pragma unroll
-        for (int i = 0; i < EPT; ++i) {
-            const int e = tid + i * NT;
-            if (NE % NT == 0 || e < NE) {
-                const int j = e / T, c = e % T, col = col0 + c;
-                const cpx<TW> v = cscale(lds[pad_pos(j, LOUT, L) * T + c], scale);
-                if constexpr (MODE == FWD_STAGE) {
-                    r[f][i] = v;
-                } else {
-                    if (col < g.ncols && (col % g.hp) < g.hvalid)
-                        out[f * g.out_fs + col_addr(col, g.out_inner, g.out_os) + j * g.out_ls] = v;
+        for (int e = 0; e < E; ++e) {
+            const int p = ci_pos_in<L, PL::first, T, NT>(tid, e);
+            v[e] = colok ? src[(long long)p * g.in_ls] : zero;
+        }
+    };
+    if constexpr (MODE == FWD_STAGE) {
+        // three fields; the kept outputs of each stay in registers (static indices)
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            load_field(f);
+            fft_reg<L, -1, T, NT, T, 1, PL>(v, lds, tw, tid);
+#pragma unroll
+            for (int b = 0; b < E / RP; ++b)
+#pragma unroll
+                for (int mk = 0; mk < KR; ++mk) {
+                    const int m = mk < RP / 3 ? mk : mk + RP / 3;
+                    r[f][b * KR + mk] = cscale(v[b * RP + m], scale);
+                }
+            __syncthreads();
+        }
+    } else {
+        for (int f = 0; f < NF; ++f) {
+            load_field(f);
+            fft_reg<L, -1, T, NT, T, 1, PL>(v, lds, tw, tid);
+            if (colok) {
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    const int p = ci_pos_out<L, RP, T, NT>(tid, e);
+                    const int j = LOUT == L ? p : unpad_index(p, LOUT, L);
+                    if (j >= 0) out[f * g.out_fs + cout + (long long)j * g.out_ls] = cscale(v[e], scale);
                 }
             }
+            __syncthreads();
         }
     }
     if constexpr (MODE == FWD_STAGE) {
 #pragma unroll
-        for (int i = 0; i < EPT; ++i) {
-            const int e = tid + i * NT;
-            if (!(NE % NT == 0 || e < NE)) continue;
-            const int j = e / T, col = col0 + e % T;
-            if (!(col < g.ncols && (col % g.hp) < g.hvalid)) continue;
-            const int iy = col / g.hp, iz = col % g.hp;
-            const int wx = wavenum(j, n), wy = wavenum(iy, n);
-            const TW kx = TW(double(wx) * ks), ky = TW(double(wy) * ks), kz = TW(double(iz) * ks);
-            // Leray projection (vonkarman-core/src/spectral_ops.rs:114-151)
-            cpx<TW> x = r[0][i], y = r[1][i], z = r[2][i];
-            const TW k2 = kx * kx + ky * ky + kz * kz;
-            if (!(double(k2) < 1e-30)) {
-                const TW inv = TW(1) / k2;
-                const TW dr = kx * x.re + ky * y.re + kz * z.re;
-                const TW di = kx * x.im + ky * y.im + kz * z.im;
-                x = {x.re - kx * dr * inv, x.im - kx * di * inv};
-                y = {y.re - ky * dr * inv, y.im - ky * di * inv};
-                z = {z.re - kz * dr * inv, z.im - kz * di * inv};
+        for (int b = 0; b < E / RP; ++b)
+#pragma unroll
+            for (int mk = 0; mk < KR; ++mk) {
+                const int m = mk < RP / 3 ? mk : mk + RP / 3;
+                if (!colok) continue;
+                const int p = ci_pos_out<L, RP, T, NT>(tid, b * RP + m);
+                const int j = unpad_index(p, LOUT, L);
+                const int iy = col / g.hp, iz = col % g.hp;
+                const int wx = wavenum(j, n), wy = wavenum(iy, n);
+                const TW kx = TW(double(wx) * ks), ky = TW(double(wy) * ks), kz = TW(double(iz) * ks);
+                // Leray projection (vonkarman-core/src/spectral_ops.rs:114-151)
+                cpx<TW> x = r[0][b * KR + mk], y = r[1][b * KR + mk], z = r[2][b * KR + mk];
+                const TW k2 = kx * kx + ky * ky + kz * kz;
+                if (!(double(k2) < 1e-30)) {
+                    const TW inv = TW(1) / k2;
+                    const TW dr = kx * x.re + ky * y.re + kz * z.re;
+                    const TW di = kx * x.im + ky * y.im + kz * z.im;
+                    x = {x.re - kx * dr * inv, x.im - kx * di * inv};
+                    y = {y.re - ky * dr * inv, y.im - ky * di * inv};
+                    z = {z.re - kz * dr * inv, z.im - kz * di * inv};
+                }
+                const cpx<double> nn[3] = {cvt<double>(x), cvt<double>(y), cvt<double>(z)};
+                const long long idx = cout + (long long)j * g.out_ls;
+                stage_epilogue<double>(sa, idx, wx * wx + wy * wy + iz * iz, nn);
             }
-            const cpx<double> nn[3] = {cvt<double>(x), cvt<double>(y), cvt<double>(z)};
-            const long long idx = col_addr(col, g.out_inner, g.out_os) + j * g.out_ls;
-            stage_epilogue<double>(sa, idx, wx * wx + wy * wy + iz * iz, nn);
-        }
     }
 }
 
@@ -264,90 +308,104 @@ __global__ __launch_bounds__((TileCfg<L, TW>::NT)) void k_fwd(const cpx<TW> *__r
 //   C2R:  Z_k = X_k + i Y_k,  Z_{L-k} = conj(X_k) + i conj(Y_k)  ->  x + i y
 //   R2C:  A_k = (Z_k + conj Z_{L-k})/2,  B_k = (Z_k - conj Z_{L-k})/(2i)
 // The imaginary parts of the k=0 and k=L/2 inputs are ignored by the C2R
-// (vonkarman-fft/src/backend.rs:15-18 semantics).
+// (vonkarman-fft/src/backend.rs:15-18 semantics).  Each block holds T lines
+// (line-major tile, pitch L); registers are filled straight from HBM.
 // ===========================================================================
-template <int L, int HIN, int TLINES, int LP, typename TW>
-__device__ __forceinline__ void z_build_pair(cpx<TW> *lds, const cpx<TW> *__restrict__ fa,
-                                             const cpx<TW> *__restrict__ fb, long long line0,
-                                             long long lines, int hp, int tid, int nt) {
-    const cpx<TW> zero = {TW(0), TW(0)};
-    // zero the gap between the positive and mirrored negative modes
-    constexpr int GAP0 = HIN, GAP = L - 2 * HIN + 1;
-    if constexpr (GAP > 0) {
-        for (int e = tid; e < TLINES * GAP; e += nt) lds[(e / GAP) * LP + GAP0 + e % GAP] = zero;
+template <int L, int E>
+struct ZGeom {
+    static constexpr int NT = 128;
+    static constexpr int T = NT * E / L;
+    static_assert(T >= 1 && T * L == NT * E, "z tile must split evenly");
+};
+
+// Hermitian-extended pair spectrum Z at FFT position p from the half
+// spectra fa, fb (HIN stored entries; fb may be null)
+template <int L, int HIN, typename TW>
+__device__ __forceinline__ cpx<TW> z_pair_at(const cpx<TW> *__restrict__ fa, const cpx<TW> *__restrict__ fb,
+                                            long long base, int p) {
+    if (p == 0 || 2 * p == L) {
+        if (p >= HIN) return {TW(0), TW(0)};
+        const cpx<TW> X = fa[base + p];
+        return {X.re, fb ? fb[base + p].re : TW(0)};
     }
-    for (int e = tid; e < TLINES * HIN; e += nt) {
-        const int t = e / HIN, k = e % HIN;
-        const long long line = line0 + t;
-        cpx<TW> X = zero, Y = zero;
-        if (line < lines) {
-            X = fa[line * hp + k];
-            if (fb) Y = fb[line * hp + k];
-        }
-        cpx<TW> *row = lds + t * LP;
-        if (k == 0 || 2 * k == L) {
-            row[k] = {X.re, Y.re};
-        } else {
-            row[k] = {X.re - Y.im, X.im + Y.re};
-            row[L - k] = {X.re + Y.im, Y.re - X.im};
-        }
+    if (p < HIN) {
+        const cpx<TW> X = fa[base + p];
+        const cpx<TW> Y = fb ? fb[base + p] : cpx<TW>{TW(0), TW(0)};
+        return {X.re - Y.im, X.im + Y.re};
     }
+    if (p > L - HIN) {
+        const int k = L - p;
+        const cpx<TW> X = fa[base + k];
+        const cpx<TW> Y = fb ? fb[base + k] : cpx<TW>{TW(0), TW(0)};
+        return {X.re + Y.im, Y.re - X.im};
+    }
+    return {TW(0), TW(0)};
 }
 
 template <int L, typename TW>
-__global__ __launch_bounds__((TileCfg<L, TW>::NTZ)) void k_z_nl(const cpx<TW> *__restrict__ in,
-                                                              cpx<TW> *__restrict__ out,
-                                                              long long lines, int hp, long long fs_in,
-                                                              long long fs_out, TW s_in, TW s_out,
-                                                              const cpx<TW> *__restrict__ tw) {
-    using C = TileCfg<L, TW>;
-    constexpr int TZ = C::TZ, NT = C::NTZ, LP = L;
+struct ZNLPlan {
+    using type = PlanE12<L>;
+};
+
+template <int L, typename TW>
+__global__ __launch_bounds__(128) void k_z_nl(const cpx<TW> *__restrict__ in, cpx<TW> *__restrict__ out,
+                                              long long lines, int hp, long long fs_in, long long fs_out, TW s_in,
+                                              TW s_out, const cpx<TW> *__restrict__ tw) {
+    using P = typename ZNLPlan<L, TW>::type;
+    using PL = typename P::radices;       // inverse order
+    using RL = typename Reverse<PL>::type;  // forward order: first radix = inverse's last
+    constexpr int E = P::E;
+    constexpr int T = ZGeom<L, E>::T, NT = ZGeom<L, E>::NT, LP = L;
     constexpr int HIN = L / 3 + 1;  // N/2+1 with N = 2L/3
-    constexpr int NE = TZ * L;
-    constexpr int EPT = (NE + NT - 1) / NT;
-    __shared__ cpx<TW> lds[TZ * LP];
+    __shared__ cpx<TW> lds[T * LP];
     const int tid = threadIdx.x;
-    const long long line0 = (long long)blockIdx.x * TZ;
-    TW r[6][EPT];
+    const long long line0 = (long long)blockIdx.x * T;
+    cpx<TW> v[E];
+    TW r[6][E];
 
     // fields: 0 ux, 1 uy, 2 uz, 3 wx, 4 wy, 5 wz; pairs (0,1) (2,3) (4,5)
-    for (int q = 0; q < 3; ++q) {
-        __syncthreads();
-        z_build_pair<L, HIN, TZ, LP>(lds, in + (2 * q) * fs_in, in + (2 * q + 1) * fs_in, line0, lines, hp,
-                                     tid, NT);
-        __syncthreads();
-        fft_tile<L, +1, TZ, NT, 1, LP>(lds, tw, tid);
 #pragma unroll
-        for (int i = 0; i < EPT; ++i) {
-            const int e = tid + i * NT;
-            if (NE % NT == 0 || e < NE) {
-                const cpx<TW> z = lds[(e / L) * LP + e % L];
-                r[2 * q][i] = z.re * s_in;
-                r[2 * q + 1][i] = z.im * s_in;
-            }
+    for (int q = 0; q < 3; ++q) {
+        const cpx<TW> *fa = in + (2 * q) * fs_in, *fb = in + (2 * q + 1) * fs_in;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            int t, p;
+            reg_pos_in<L, PL::first, T, NT, LP>(tid, e, t, p);
+            const long long line = line0 + t;
+            v[e] = line < lines ? z_pair_at<L, HIN>(fa, fb, line * hp, p) : cpx<TW>{TW(0), TW(0)};
         }
+        fft_reg<L, +1, T, NT, 1, LP, PL>(v, lds, tw, tid);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            r[2 * q][e] = v[e].re * s_in;
+            r[2 * q + 1][e] = v[e].im * s_in;
+        }
+        __syncthreads();
     }
     // u x w (vonkarman-fft/src/dealiased.rs:79-81)
 #pragma unroll
-    for (int i = 0; i < EPT; ++i) {
-        const TW ux = r[0][i], uy = r[1][i], uz = r[2][i];
-        const TW ox = r[3][i], oy = r[4][i], oz = r[5][i];
-        r[0][i] = uy * oz - uz * oy;
-        r[1][i] = uz * ox - ux * oz;
-        r[2][i] = ux * oy - uy * ox;
+    for (int e = 0; e < E; ++e) {
+        const TW ux = r[0][e], uy = r[1][e], uz = r[2][e];
+        const TW ox = r[3][e], oy = r[4][e], oz = r[5][e];
+        r[0][e] = uy * oz - uz * oy;
+        r[1][e] = uz * ox - ux * oz;
+        r[2][e] = ux * oy - uy * ox;
     }
     constexpr int HOUT = HIN;
-    const TW h = TW(0.5) * s_out;
-    // (c0 + i c1) -> two truncated half spectra
+    // (c0 + i c1): forward, then split through LDS (needs k and L-k)
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = {r[0][e], r[1][e]};
+    fft_reg<L, -1, T, NT, 1, LP, RL>(v, lds, tw, tid);
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < EPT; ++i) {
-        const int e = tid + i * NT;
-        if (NE % NT == 0 || e < NE) lds[(e / L) * LP + e % L] = {r[0][i], r[1][i]};
+    for (int e = 0; e < E; ++e) {
+        int t, p;
+        reg_pos_out<L, RL::last, T, NT, LP>(tid, e, t, p);
+        lds[t * LP + p] = v[e];
     }
     __syncthreads();
-    fft_tile<L, -1, TZ, NT, 1, LP>(lds, tw, tid);
-    for (int e = tid; e < TZ * HOUT; e += NT) {
+    const TW h = TW(0.5) * s_out;
+    for (int e = tid; e < T * HOUT; e += NT) {
         const int t = e / HOUT, k = e % HOUT;
         const long long line = line0 + t;
         const cpx<TW> zk = lds[t * LP + k], zm = lds[t * LP + (k ? L - k : 0)];
@@ -356,59 +414,60 @@ __global__ __launch_bounds__((TileCfg<L, TW>::NTZ)) void k_z_nl(const cpx<TW> *_
             out[1 * fs_out + line * hp + k] = {(zk.im + zm.im) * h, (zm.re - zk.re) * h};
         }
     }
-    // c2 alone
     __syncthreads();
+    // c2 alone: its first HOUT outputs are stored from registers
 #pragma unroll
-    for (int i = 0; i < EPT; ++i) {
-        const int e = tid + i * NT;
-        if (NE % NT == 0 || e < NE) lds[(e / L) * LP + e % L] = {r[2][i], TW(0)};
-    }
-    __syncthreads();
-    fft_tile<L, -1, TZ, NT, 1, LP>(lds, tw, tid);
-    for (int e = tid; e < TZ * HOUT; e += NT) {
-        const int t = e / HOUT, k = e % HOUT;
+    for (int e = 0; e < E; ++e) v[e] = {r[2][e], TW(0)};
+    fft_reg<L, -1, T, NT, 1, LP, RL>(v, lds, tw, tid);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        int t, p;
+        reg_pos_out<L, RL::last, T, NT, LP>(tid, e, t, p);
         const long long line = line0 + t;
-        if (line < lines) out[2 * fs_out + line * hp + k] = cscale(lds[t * LP + k], s_out);
+        if (p < HOUT && line < lines) out[2 * fs_out + line * hp + p] = cscale(v[e], s_out);
     }
 }
 
 // C2R of three unpadded fields and the maximum of |v|^2 (CFL speed or |w|).
 template <int L, typename TW>
-__global__ __launch_bounds__((TileCfg<L, TW>::NTZ)) void k_z_max(const cpx<TW> *__restrict__ in, long long lines,
-                                                               int hp, long long fs, TW s_in,
-                                                               const cpx<TW> *__restrict__ tw, Scalars *sc) {
-    using C = TileCfg<L, TW>;
-    constexpr int TZ = C::TZ, NT = C::NTZ, LP = L;
+__global__ __launch_bounds__(128) void k_z_max(const cpx<TW> *__restrict__ in, long long lines, int hp,
+                                              long long fs, TW s_in, const cpx<TW> *__restrict__ tw, Scalars *sc) {
+    using P = typename PlanSmall<L>::type;
+    using PL = typename P::radices;
+    constexpr int E = P::E;
+    constexpr int T = ZGeom<L, E>::T, NT = ZGeom<L, E>::NT, LP = L;
     constexpr int HIN = L / 2 + 1;
-    constexpr int NE = TZ * L;
-    constexpr int EPT = (NE + NT - 1) / NT;
-    __shared__ cpx<TW> lds[TZ * LP];
+    __shared__ cpx<TW> lds[T * LP];
     __shared__ double wmax[NT / 64];
     const int tid = threadIdx.x;
-    const long long line0 = (long long)blockIdx.x * TZ;
-    TW r[3][EPT];
-    for (int q = 0; q < 2; ++q) {
-        __syncthreads();
-        z_build_pair<L, HIN, TZ, LP>(lds, in + (2 * q) * fs, q == 0 ? in + fs : nullptr, line0, lines, hp, tid,
-                                     NT);
-        __syncthreads();
-        fft_tile<L, +1, TZ, NT, 1, LP>(lds, tw, tid);
+    const long long line0 = (long long)blockIdx.x * T;
+    cpx<TW> v[E];
+    TW r[3][E];
 #pragma unroll
-        for (int i = 0; i < EPT; ++i) {
-            const int e = tid + i * NT;
-            if (NE % NT == 0 || e < NE) {
-                const cpx<TW> z = lds[(e / L) * LP + e % L];
-                r[2 * q][i] = z.re * s_in;
-                if (q == 0) r[1][i] = z.im * s_in;
-            }
+    for (int q = 0; q < 2; ++q) {
+        const cpx<TW> *fa = in + (2 * q) * fs, *fb = q == 0 ? in + fs : nullptr;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            int t, p;
+            reg_pos_in<L, PL::first, T, NT, LP>(tid, e, t, p);
+            const long long line = line0 + t;
+            v[e] = line < lines ? z_pair_at<L, HIN>(fa, fb, line * hp, p) : cpx<TW>{TW(0), TW(0)};
         }
+        fft_reg<L, +1, T, NT, 1, LP, PL>(v, lds, tw, tid);
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            r[2 * q][e] = v[e].re * s_in;
+            if (q == 0) r[1][e] = v[e].im * s_in;
+        }
+        __syncthreads();
     }
     double m = 0.0;
 #pragma unroll
-    for (int i = 0; i < EPT; ++i) {
-        const int e = tid + i * NT;
-        if ((NE % NT == 0 || e < NE) && line0 + e / L < lines) {
-            const double a = r[0][i], b = r[1][i], c = r[2][i];
+    for (int e = 0; e < E; ++e) {
+        int t, p;
+        reg_pos_out<L, PL::last, T, NT, LP>(tid, e, t, p);
+        if (line0 + t < lines) {
+            const double a = r[0][e], b = r[1][e], c = r[2][e];
             m = fmax(m, a * a + b * b + c * c);
         }
     }
@@ -421,64 +480,78 @@ __global__ __launch_bounds__((TileCfg<L, TW>::NTZ)) void k_z_max(const cpx<TW> *
     }
 }
 
-// standalone C2R of one field along the contiguous axis: lines (2s, 2s+1)
-// share one complex DFT.
+// standalone C2R of one field along the contiguous axis: data lines
+// (2s, 2s+1) share transform line s.
 template <int L, typename TW>
-__global__ __launch_bounds__((TileCfg<L, TW>::NTZ)) void k_z_c2r(const cpx<TW> *__restrict__ in,
-                                                               double *__restrict__ out, long long lines, int hp,
-                                                               TW scale, const cpx<TW> *__restrict__ tw) {
-    using C = TileCfg<L, TW>;
-    constexpr int TZ = C::TZ, NT = C::NTZ, LP = L, NS = TZ / 2;
+__global__ __launch_bounds__(128) void k_z_c2r(const cpx<TW> *__restrict__ in, double *__restrict__ out,
+                                              long long lines, int hp, TW scale, const cpx<TW> *__restrict__ tw) {
+    using P = PlanS<L>;
+    using PL = typename P::radices;
+    constexpr int E = P::E;
+    constexpr int T = ZGeom<L, E>::T, NT = ZGeom<L, E>::NT, LP = L;
     constexpr int HIN = L / 2 + 1;
-    __shared__ cpx<TW> lds[NS * LP];
+    __shared__ cpx<TW> lds[T * LP];
     const int tid = threadIdx.x;
-    const long long line0 = (long long)blockIdx.x * TZ;
-    const cpx<TW> zero = {TW(0), TW(0)};
-    for (int e = tid; e < NS * HIN; e += NT) {
-        const int s = e / HIN, k = e % HIN;
-        const long long la = line0 + 2 * s, lb = la + 1;
-        const cpx<TW> X = la < lines ? in[la * hp + k] : zero;
-        const cpx<TW> Y = lb < lines ? in[lb * hp + k] : zero;
-        cpx<TW> *row = lds + s * LP;
-        if (k == 0 || 2 * k == L) {
-            row[k] = {X.re, Y.re};
-        } else {
-            row[k] = {X.re - Y.im, X.im + Y.re};
-            row[L - k] = {X.re + Y.im, Y.re - X.im};
+    const long long pair0 = (long long)blockIdx.x * T;
+    cpx<TW> v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        int s, p;
+        reg_pos_in<L, PL::first, T, NT, LP>(tid, e, s, p);
+        const long long la = 2 * (pair0 + s), lb = la + 1;
+        v[e] = {TW(0), TW(0)};
+        if (la < lines) {
+            // the partner line beyond the end reads as zero
+            const cpx<TW> *fb = lb < lines ? in : nullptr;
+            const cpx<TW> za = z_pair_at<L, HIN>(in, (const cpx<TW> *)nullptr, la * hp, p);
+            const cpx<TW> zb = fb ? z_pair_at<L, HIN>(in, (const cpx<TW> *)nullptr, lb * hp, p) : cpx<TW>{TW(0), TW(0)};
+            // za = X_a (Hermitian-extended), zb = X_b: Z = za + i zb
+            v[e] = {za.re - zb.im, za.im + zb.re};
         }
     }
-    __syncthreads();
-    fft_tile<L, +1, NS, NT, 1, LP>(lds, tw, tid);
-    for (int e = tid; e < NS * L; e += NT) {
-        const int s = e / L, nn = e % L;
-        const long long la = line0 + 2 * s, lb = la + 1;
-        const cpx<TW> z = lds[s * LP + nn];
-        if (la < lines) out[la * L + nn] = double(z.re * scale);
-        if (lb < lines) out[lb * L + nn] = double(z.im * scale);
+    fft_reg<L, +1, T, NT, 1, LP, PL>(v, lds, tw, tid);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        int s, p;
+        reg_pos_out<L, PL::last, T, NT, LP>(tid, e, s, p);
+        const long long la = 2 * (pair0 + s), lb = la + 1;
+        if (la < lines) out[la * L + p] = double(v[e].re * scale);
+        if (lb < lines) out[lb * L + p] = double(v[e].im * scale);
     }
 }
 
 template <int L, typename TW>
-__global__ __launch_bounds__((TileCfg<L, TW>::NTZ)) void k_z_r2c(const double *__restrict__ in,
-                                                               cpx<TW> *__restrict__ out, long long lines, int hp,
-                                                               const cpx<TW> *__restrict__ tw) {
-    using C = TileCfg<L, TW>;
-    constexpr int TZ = C::TZ, NT = C::NTZ, LP = L, NS = TZ / 2;
+__global__ __launch_bounds__(128) void k_z_r2c(const double *__restrict__ in, cpx<TW> *__restrict__ out,
+                                              long long lines, int hp, const cpx<TW> *__restrict__ tw) {
+    using P = PlanS<L>;
+    using PL = typename P::radices;
+    constexpr int E = P::E;
+    constexpr int T = ZGeom<L, E>::T, NT = ZGeom<L, E>::NT, LP = L;
     constexpr int HOUT = L / 2 + 1;
-    __shared__ cpx<TW> lds[NS * LP];
+    __shared__ cpx<TW> lds[T * LP];
     const int tid = threadIdx.x;
-    const long long line0 = (long long)blockIdx.x * TZ;
-    for (int e = tid; e < NS * L; e += NT) {
-        const int s = e / L, nn = e % L;
-        const long long la = line0 + 2 * s, lb = la + 1;
-        lds[s * LP + nn] = {la < lines ? TW(in[la * L + nn]) : TW(0), lb < lines ? TW(in[lb * L + nn]) : TW(0)};
+    const long long pair0 = (long long)blockIdx.x * T;
+    cpx<TW> v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        int s, p;
+        reg_pos_in<L, PL::first, T, NT, LP>(tid, e, s, p);
+        const long long la = 2 * (pair0 + s), lb = la + 1;
+        v[e] = {la < lines ? TW(in[la * L + p]) : TW(0), lb < lines ? TW(in[lb * L + p]) : TW(0)};
+    }
+    fft_reg<L, -1, T, NT, 1, LP, PL>(v, lds, tw, tid);
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        int s, p;
+        reg_pos_out<L, PL::last, T, NT, LP>(tid, e, s, p);
+        lds[s * LP + p] = v[e];
     }
     __syncthreads();
-    fft_tile<L, -1, NS, NT, 1, LP>(lds, tw, tid);
     const TW h = TW(0.5);
-    for (int e = tid; e < NS * HOUT; e += NT) {
+    for (int e = tid; e < T * HOUT; e += NT) {
         const int s = e / HOUT, k = e % HOUT;
-        const long long la = line0 + 2 * s, lb = la + 1;
+        const long long la = 2 * (pair0 + s), lb = la + 1;
         const cpx<TW> zk = lds[s * LP + k], zm = lds[s * LP + (k ? L - k : 0)];
         if (la < lines) out[la * hp + k] = {(zk.re + zm.re) * h, (zk.im - zm.im) * h};
         if (lb < lines) out[lb * hp + k] = {(zk.im + zm.im) * h, (zm.re - zk.re) * h};
@@ -496,7 +569,7 @@ inline dim3 blocks_for(long long n, int per) {
 template <int L, int LIN, int MODE, typename TI, typename TW>
 int run_inv(const void *in, void *out, const void *tw, const Strided &g, int n, double ks, double scale,
             hipStream_t st) {
-    using C = TileCfg<L, TW>;
+    using C = InvGeom<L, MODE, TW>;
     const unsigned nb = (unsigned)((g.ncols + C::T - 1) / C::T);
     hipLaunchKernelGGL((k_inv<L, LIN, MODE, TI, TW>), dim3(nb), dim3(C::NT), 0, st,
                        reinterpret_cast<const cpx<TI> *>(in), reinterpret_cast<cpx<TW> *>(out),
@@ -545,7 +618,7 @@ int launch_inv_L(int mode, int lin, Prec ip, Prec wp, const void *in, void *out,
 template <int L, int LOUT, int MODE, int NF, typename TW>
 int run_fwd(const void *in, void *out, const void *tw, const Strided &g, int n, double ks, double scale,
             const StageArgs *sa, hipStream_t st) {
-    using C = TileCfg<L, TW>;
+    using C = FwdGeom<L, MODE, TW>;
     const unsigned nb = (unsigned)((g.ncols + C::T - 1) / C::T);
     StageArgs a{};
     if (sa) a = *sa;
@@ -583,13 +656,13 @@ int launch_z_nl_L(Prec wp, const void *in, void *out, long long lines, int hp, l
                   double s_in, double s_out, const void *tw, hipStream_t st) {
     if constexpr (L % 3 == 0 && (2 * L / 3) % 2 == 0) {
         if (wp == F64) {
-            using C = TileCfg<L, double>;
-            hipLaunchKernelGGL((k_z_nl<L, double>), blocks_for<void>(lines, C::TZ), dim3(C::NTZ), 0, st,
+            using G = ZGeom<L, ZNLPlan<L, double>::type::E>;
+            hipLaunchKernelGGL((k_z_nl<L, double>), blocks_for<void>(lines, G::T), dim3(G::NT), 0, st,
                                reinterpret_cast<const cpx<double> *>(in), reinterpret_cast<cpx<double> *>(out), lines,
                                hp, fs_in, fs_out, s_in, s_out, reinterpret_cast<const cpx<double> *>(tw));
         } else {
-            using C = TileCfg<L, float>;
-            hipLaunchKernelGGL((k_z_nl<L, float>), blocks_for<void>(lines, C::TZ), dim3(C::NTZ), 0, st,
+            using G = ZGeom<L, ZNLPlan<L, float>::type::E>;
+            hipLaunchKernelGGL((k_z_nl<L, float>), blocks_for<void>(lines, G::T), dim3(G::NT), 0, st,
                                reinterpret_cast<const cpx<float> *>(in), reinterpret_cast<cpx<float> *>(out), lines,
                                hp, fs_in, fs_out, float(s_in), float(s_out), reinterpret_cast<const cpx<float> *>(tw));
         }
@@ -601,14 +674,13 @@ int launch_z_nl_L(Prec wp, const void *in, void *out, long long lines, int hp, l
 template <int L>
 int launch_z_max_L(Prec wp, const void *in, long long lines, int hp, long long fs, double s_in, const void *tw,
                    Scalars *sc, hipStream_t st) {
+    using G = ZGeom<L, PlanSmall<L>::type::E>;
     if (wp == F64) {
-        using C = TileCfg<L, double>;
-        hipLaunchKernelGGL((k_z_max<L, double>), blocks_for<void>(lines, C::TZ), dim3(C::NTZ), 0, st,
+        hipLaunchKernelGGL((k_z_max<L, double>), blocks_for<void>(lines, G::T), dim3(G::NT), 0, st,
                            reinterpret_cast<const cpx<double> *>(in), lines, hp, fs, s_in,
                            reinterpret_cast<const cpx<double> *>(tw), sc);
     } else {
-        using C = TileCfg<L, float>;
-        hipLaunchKernelGGL((k_z_max<L, float>), blocks_for<void>(lines, C::TZ), dim3(C::NTZ), 0, st,
+        hipLaunchKernelGGL((k_z_max<L, float>), blocks_for<void>(lines, G::T), dim3(G::NT), 0, st,
                            reinterpret_cast<const cpx<float> *>(in), lines, hp, fs, float(s_in),
                            reinterpret_cast<const cpx<float> *>(tw), sc);
     }
@@ -618,8 +690,8 @@ int launch_z_max_L(Prec wp, const void *in, long long lines, int hp, long long f
 template <int L>
 int launch_z_c2r_L(const void *in, double *out, long long lines, int hp, double scale, const void *tw,
                    hipStream_t st) {
-    using C = TileCfg<L, double>;
-    hipLaunchKernelGGL((k_z_c2r<L, double>), blocks_for<void>(lines, C::TZ), dim3(C::NTZ), 0, st,
+    using G = ZGeom<L, PlanS<L>::E>;
+    hipLaunchKernelGGL((k_z_c2r<L, double>), blocks_for<void>((lines + 1) / 2, G::T), dim3(G::NT), 0, st,
                        reinterpret_cast<const cpx<double> *>(in), out, lines, hp, scale,
                        reinterpret_cast<const cpx<double> *>(tw));
     return hipGetLastError() == hipSuccess ? 0 : 3;
@@ -627,8 +699,8 @@ int launch_z_c2r_L(const void *in, double *out, long long lines, int hp, double 
 
 template <int L>
 int launch_z_r2c_L(const double *in, void *out, long long lines, int hp, const void *tw, hipStream_t st) {
-    using C = TileCfg<L, double>;
-    hipLaunchKernelGGL((k_z_r2c<L, double>), blocks_for<void>(lines, C::TZ), dim3(C::NTZ), 0, st, in,
+    using G = ZGeom<L, PlanS<L>::E>;
+    hipLaunchKernelGGL((k_z_r2c<L, double>), blocks_for<void>((lines + 1) / 2, G::T), dim3(G::NT), 0, st, in,
                        reinterpret_cast<cpx<double> *>(out), lines, hp, reinterpret_cast<const cpx<double> *>(tw));
     return hipGetLastError() == hipSuccess ? 0 : 3;
 }
