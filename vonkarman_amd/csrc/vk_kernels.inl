@@ -84,58 +84,24 @@ __global__ __launch_bounds__((InvGeom<L, MODE, TW>::NT)) void k_inv(const cpx<TI
         const int p = ci_pos_in<L, PL::first, T, NT>(tid, e);
         return LIN == L ? p : unpad_index(p, LIN, L);
     };
-    const cpx<TI> *src = in;
-    auto load = [&](int f, int e) -> cpx<TW> {
-        const int i = uidx(e);
-        if (!colok || i < 0) return zero;
-        return cvt<TW>(src[f * g.in_fs + cin + (long long)i * g.in_ls]);
+    cpx<TW> A[E], B[E];
+    // fill a register slot with input field f (opaque pointer: the loads stay
+    // after the previous transform's barrier)
+    auto fill = [&](cpx<TW>(&S)[E], int f) {
+        const cpx<TI> *src = opaque(in) + f * g.in_fs + cin;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int i = uidx(e);
+            S[e] = (colok && i >= 0) ? cvt<TW>(src[(long long)i * g.in_ls]) : zero;
+        }
     };
     auto kline = [&](int e) -> TW {
         const int i = uidx(e);
         return TW(double(wavenum(i < 0 ? 0 : i, LIN == L ? n : LIN)) * ks);
     };
-    auto kcol = [&](int) -> TW { return kz_col; };
-    auto ik = [](TW k, cpx<TW> a) -> cpx<TW> { return {-k * a.im, k * a.re}; };
-    // outputs by mode:
-    //   PLAIN: 0 <- in 0;  U3: f <- in f
-    //   U3DX (x pass): 0 ux, 1 uy, 2 uz, 3 i kx uz, 4 i kx uy
-    //   UOMEGA (y pass, inputs 0 ux, 1 uy, 2 uz, 3 ax = i kx uz, 4 bx = i kx uy):
-    //     0..2 u, 3 wx = i (ky uz - kz uy), 4 wy = i kz ux - ax, 5 wz = bx - i ky ux
-    //   OMEGA: the three w outputs only
-    constexpr int NO = MODE == INV_PLAIN ? 1 : MODE == INV_U3 ? 3 : MODE == INV_U3DX ? 5 : MODE == INV_OMEGA ? 3 : 6;
-#pragma unroll 1
-    for (int fo = 0; fo < NO; ++fo) {
-        src = opaque(in);
-        // kind of output fo: 0..2 plain copy of input fo; 3 i kx uz; 4 i kx uy; 5 wx; 6 wy; 7 wz
-        int kind = fo;
-        if constexpr (MODE == INV_OMEGA) kind = 5 + fo;
-        if constexpr (MODE == INV_UOMEGA) kind = fo < 3 ? fo : 2 + fo;
+    auto emit = [&](int fo, auto value) {
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            switch (kind) {
-                case 3: v[e] = ik(kline(e), load(2, e)); break;
-                case 4: v[e] = ik(kline(e), load(1, e)); break;
-                case 5: {
-                    const TW ky = kline(e), kz = kcol(e);
-                    const cpx<TW> uy = load(1, e), uz = load(2, e);
-                    v[e] = ik(TW(1), cpx<TW>{ky * uz.re - kz * uy.re, ky * uz.im - kz * uy.im});
-                    break;
-                }
-                case 6: {
-                    const TW kz = kcol(e);
-                    const cpx<TW> ux = load(0, e), ax = load(3, e);
-                    v[e] = {-kz * ux.im - ax.re, kz * ux.re - ax.im};
-                    break;
-                }
-                case 7: {
-                    const TW ky = kline(e);
-                    const cpx<TW> ux = load(0, e), bx = load(4, e);
-                    v[e] = {bx.re + ky * ux.im, bx.im - ky * ux.re};
-                    break;
-                }
-                default: v[e] = load(kind, e); break;
-            }
-        }
+        for (int e = 0; e < E; ++e) v[e] = value(e);
         fft_reg<L, +1, T, NT, T, 1, PL>(v, lds, tw, tid);
         if (colok) {
 #pragma unroll
@@ -145,6 +111,54 @@ __global__ __launch_bounds__((InvGeom<L, MODE, TW>::NT)) void k_inv(const cpx<TI
             }
         }
         __syncthreads();
+    };
+    auto ik = [](TW k, cpx<TW> a) -> cpx<TW> { return {-k * a.im, k * a.re}; };
+    auto copyA = [&](int e) { return A[e]; };
+    auto copyB = [&](int e) { return B[e]; };
+    // y pass curl terms; inputs 0 ux, 1 uy, 2 uz, 3 ax = i kx uz, 4 bx = i kx uy
+    auto wx = [&](int e) {  // i (ky uz - kz uy), A = uy, B = uz
+        const TW ky = kline(e);
+        return ik(TW(1), cpx<TW>{ky * B[e].re - kz_col * A[e].re, ky * B[e].im - kz_col * A[e].im});
+    };
+    auto wy = [&](int e) {  // i kz ux - ax, A = ux, B = ax
+        return cpx<TW>{-kz_col * A[e].im - B[e].re, kz_col * A[e].re - B[e].im};
+    };
+    auto wz = [&](int e) {  // bx - i ky ux, A = ux, B = bx
+        const TW ky = kline(e);
+        return cpx<TW>{B[e].re + ky * A[e].im, B[e].im - ky * A[e].re};
+    };
+
+    if constexpr (MODE == INV_PLAIN || MODE == INV_U3) {
+        constexpr int NF = MODE == INV_PLAIN ? 1 : 3;
+#pragma unroll 1
+        for (int f = 0; f < NF; ++f) {
+            fill(A, f);
+            emit(f, copyA);
+        }
+    } else if constexpr (MODE == INV_U3DX) {
+        // outputs: 0 ux, 1 uy, 2 uz, 3 i kx uz, 4 i kx uy
+        fill(A, 0);
+        emit(0, copyA);
+        fill(A, 1);
+        emit(1, copyA);
+        emit(4, [&](int e) { return ik(kline(e), A[e]); });
+        fill(A, 2);
+        emit(2, copyA);
+        emit(3, [&](int e) { return ik(kline(e), A[e]); });
+    } else {
+        // UOMEGA outputs 0..2 u, 3 wx, 4 wy, 5 wz; OMEGA outputs 0 wx, 1 wy, 2 wz
+        constexpr int OW = (MODE == INV_UOMEGA) ? 3 : 0;
+        fill(A, 0);
+        if constexpr (MODE == INV_UOMEGA) emit(0, copyA);
+        fill(B, 3);
+        emit(OW + 1, wy);
+        fill(B, 4);
+        emit(OW + 2, wz);
+        fill(A, 1);
+        if constexpr (MODE == INV_UOMEGA) emit(1, copyA);
+        fill(B, 2);
+        if constexpr (MODE == INV_UOMEGA) emit(2, copyB);
+        emit(OW + 0, wx);
     }
 }
 
@@ -311,13 +325,6 @@ __global__ __launch_bounds__((FwdGeom<L, MODE, TW>::NT)) void k_fwd(const cpx<TW
 // (vonkarman-fft/src/backend.rs:15-18 semantics).  Each block holds T lines
 // (line-major tile, pitch L); registers are filled straight from HBM.
 // ===========================================================================
-template <int L, int E>
-struct ZGeom {
-    static constexpr int NT = 128;
-    static constexpr int T = NT * E / L;
-    static_assert(T >= 1 && T * L == NT * E, "z tile must split evenly");
-};
-
 // Hermitian-extended pair spectrum Z at FFT position p from the half
 // spectra fa, fb (HIN stored entries; fb may be null)
 template <int L, int HIN, typename TW>
@@ -342,39 +349,44 @@ __device__ __forceinline__ cpx<TW> z_pair_at(const cpx<TW> *__restrict__ fa, con
     return {TW(0), TW(0)};
 }
 
+// z tiles are column-interleaved too: the T lines of a block are the columns,
+// so position p of all T lines is one contiguous LDS row (no bank conflicts)
+// and each line still reads HBM in 128-byte runs across a wavefront.
 template <int L, typename TW>
-struct ZNLPlan {
-    using type = PlanE12<L>;
-};
+using ZNLP = PlanE12<L>;
+template <int L, typename TW>
+using ZNLGeom = Geom<L, ZNLP<L, TW>::E, TW>;
 
 template <int L, typename TW>
-__global__ __launch_bounds__(128) void k_z_nl(const cpx<TW> *__restrict__ in, cpx<TW> *__restrict__ out,
-                                              long long lines, int hp, long long fs_in, long long fs_out, TW s_in,
-                                              TW s_out, const cpx<TW> *__restrict__ tw) {
-    using P = typename ZNLPlan<L, TW>::type;
-    using PL = typename P::radices;       // inverse order
+__global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) void k_z_nl(const cpx<TW> *__restrict__ in,
+                                                               cpx<TW> *__restrict__ out, long long lines, int hp,
+                                                               long long fs_in, long long fs_out, TW s_in, TW s_out,
+                                                               const cpx<TW> *__restrict__ tw) {
+    using P = ZNLP<L, TW>;
+    using PL = typename P::radices;         // inverse order
     using RL = typename Reverse<PL>::type;  // forward order: first radix = inverse's last
     constexpr int E = P::E;
-    constexpr int T = ZGeom<L, E>::T, NT = ZGeom<L, E>::NT, LP = L;
+    constexpr int T = ZNLGeom<L, TW>::T, NT = ZNLGeom<L, TW>::NT;
     constexpr int HIN = L / 3 + 1;  // N/2+1 with N = 2L/3
-    __shared__ cpx<TW> lds[T * LP];
+    __shared__ cpx<TW> lds[L * T];
     const int tid = threadIdx.x;
     const long long line0 = (long long)blockIdx.x * T;
+    const int t = tid % T;
+    const bool ok = line0 + t < lines;
+    const long long base = (line0 + t) * hp;
     cpx<TW> v[E];
     TW r[6][E];
 
     // fields: 0 ux, 1 uy, 2 uz, 3 wx, 4 wy, 5 wz; pairs (0,1) (2,3) (4,5)
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-        const cpx<TW> *fa = in + (2 * q) * fs_in, *fb = in + (2 * q + 1) * fs_in;
+        const cpx<TW> *fa = opaque(in) + (2 * q) * fs_in, *fb = in + (2 * q + 1) * fs_in;
 #pragma unroll
         for (int e = 0; e < E; ++e) {
-            int t, p;
-            reg_pos_in<L, PL::first, T, NT, LP>(tid, e, t, p);
-            const long long line = line0 + t;
-            v[e] = line < lines ? z_pair_at<L, HIN>(fa, fb, line * hp, p) : cpx<TW>{TW(0), TW(0)};
+            const int p = ci_pos_in<L, PL::first, T, NT>(tid, e);
+            v[e] = ok ? z_pair_at<L, HIN>(fa, fb, base, p) : cpx<TW>{TW(0), TW(0)};
         }
-        fft_reg<L, +1, T, NT, 1, LP, PL>(v, lds, tw, tid);
+        fft_reg<L, +1, T, NT, T, 1, PL>(v, lds, tw, tid);
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             r[2 * q][e] = v[e].re * s_in;
@@ -395,20 +407,16 @@ __global__ __launch_bounds__(128) void k_z_nl(const cpx<TW> *__restrict__ in, cp
     // (c0 + i c1): forward, then split through LDS (needs k and L-k)
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = {r[0][e], r[1][e]};
-    fft_reg<L, -1, T, NT, 1, LP, RL>(v, lds, tw, tid);
+    fft_reg<L, -1, T, NT, T, 1, RL>(v, lds, tw, tid);
     __syncthreads();
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-        int t, p;
-        reg_pos_out<L, RL::last, T, NT, LP>(tid, e, t, p);
-        lds[t * LP + p] = v[e];
-    }
+    for (int e = 0; e < E; ++e) lds[ci_pos_out<L, RL::last, T, NT>(tid, e) * T + t] = v[e];
     __syncthreads();
     const TW h = TW(0.5) * s_out;
     for (int e = tid; e < T * HOUT; e += NT) {
-        const int t = e / HOUT, k = e % HOUT;
-        const long long line = line0 + t;
-        const cpx<TW> zk = lds[t * LP + k], zm = lds[t * LP + (k ? L - k : 0)];
+        const int tt = e % T, k = e / T;
+        const long long line = line0 + tt;
+        const cpx<TW> zk = lds[k * T + tt], zm = lds[(k ? L - k : 0) * T + tt];
         if (line < lines) {
             out[0 * fs_out + line * hp + k] = {(zk.re + zm.re) * h, (zk.im - zm.im) * h};
             out[1 * fs_out + line * hp + k] = {(zk.im + zm.im) * h, (zm.re - zk.re) * h};
@@ -418,42 +426,47 @@ __global__ __launch_bounds__(128) void k_z_nl(const cpx<TW> *__restrict__ in, cp
     // c2 alone: its first HOUT outputs are stored from registers
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = {r[2][e], TW(0)};
-    fft_reg<L, -1, T, NT, 1, LP, RL>(v, lds, tw, tid);
+    fft_reg<L, -1, T, NT, T, 1, RL>(v, lds, tw, tid);
+    if (ok) {
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-        int t, p;
-        reg_pos_out<L, RL::last, T, NT, LP>(tid, e, t, p);
-        const long long line = line0 + t;
-        if (p < HOUT && line < lines) out[2 * fs_out + line * hp + p] = cscale(v[e], s_out);
+        for (int e = 0; e < E; ++e) {
+            const int p = ci_pos_out<L, RL::last, T, NT>(tid, e);
+            if (p < HOUT) out[2 * fs_out + base + p] = cscale(v[e], s_out);
+        }
     }
 }
 
 // C2R of three unpadded fields and the maximum of |v|^2 (CFL speed or |w|).
 template <int L, typename TW>
-__global__ __launch_bounds__(128) void k_z_max(const cpx<TW> *__restrict__ in, long long lines, int hp,
-                                              long long fs, TW s_in, const cpx<TW> *__restrict__ tw, Scalars *sc) {
+using ZMaxGeom = Geom<L, PlanSmall<L>::type::E, TW>;
+
+template <int L, typename TW>
+__global__ __launch_bounds__((ZMaxGeom<L, TW>::NT)) void k_z_max(const cpx<TW> *__restrict__ in, long long lines,
+                                                                 int hp, long long fs, TW s_in,
+                                                                 const cpx<TW> *__restrict__ tw, Scalars *sc) {
     using P = typename PlanSmall<L>::type;
     using PL = typename P::radices;
     constexpr int E = P::E;
-    constexpr int T = ZGeom<L, E>::T, NT = ZGeom<L, E>::NT, LP = L;
+    constexpr int T = ZMaxGeom<L, TW>::T, NT = ZMaxGeom<L, TW>::NT;
     constexpr int HIN = L / 2 + 1;
-    __shared__ cpx<TW> lds[T * LP];
+    __shared__ cpx<TW> lds[L * T];
     __shared__ double wmax[NT / 64];
     const int tid = threadIdx.x;
     const long long line0 = (long long)blockIdx.x * T;
+    const int t = tid % T;
+    const bool ok = line0 + t < lines;
+    const long long base = (line0 + t) * hp;
     cpx<TW> v[E];
     TW r[3][E];
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
-        const cpx<TW> *fa = in + (2 * q) * fs, *fb = q == 0 ? in + fs : nullptr;
+        const cpx<TW> *fa = opaque(in) + (2 * q) * fs, *fb = q == 0 ? in + fs : nullptr;
 #pragma unroll
         for (int e = 0; e < E; ++e) {
-            int t, p;
-            reg_pos_in<L, PL::first, T, NT, LP>(tid, e, t, p);
-            const long long line = line0 + t;
-            v[e] = line < lines ? z_pair_at<L, HIN>(fa, fb, line * hp, p) : cpx<TW>{TW(0), TW(0)};
+            const int p = ci_pos_in<L, PL::first, T, NT>(tid, e);
+            v[e] = ok ? z_pair_at<L, HIN>(fa, fb, base, p) : cpx<TW>{TW(0), TW(0)};
         }
-        fft_reg<L, +1, T, NT, 1, LP, PL>(v, lds, tw, tid);
+        fft_reg<L, +1, T, NT, T, 1, PL>(v, lds, tw, tid);
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             r[2 * q][e] = v[e].re * s_in;
@@ -462,11 +475,9 @@ __global__ __launch_bounds__(128) void k_z_max(const cpx<TW> *__restrict__ in, l
         __syncthreads();
     }
     double m = 0.0;
+    if (ok) {
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-        int t, p;
-        reg_pos_out<L, PL::last, T, NT, LP>(tid, e, t, p);
-        if (line0 + t < lines) {
+        for (int e = 0; e < E; ++e) {
             const double a = r[0][e], b = r[1][e], c = r[2][e];
             m = fmax(m, a * a + b * b + c * c);
         }
@@ -480,81 +491,77 @@ __global__ __launch_bounds__(128) void k_z_max(const cpx<TW> *__restrict__ in, l
     }
 }
 
-// standalone C2R of one field along the contiguous axis: data lines
-// (2s, 2s+1) share transform line s.
+// standalone C2R / R2C of one field along the contiguous axis: data lines
+// (2s, 2s+1) share transform line s; a block holds T transform lines.
 template <int L, typename TW>
-__global__ __launch_bounds__(128) void k_z_c2r(const cpx<TW> *__restrict__ in, double *__restrict__ out,
-                                              long long lines, int hp, TW scale, const cpx<TW> *__restrict__ tw) {
+using ZSGeom = Geom<L, PlanS<L>::E, TW>;
+
+template <int L, typename TW>
+__global__ __launch_bounds__((ZSGeom<L, TW>::NT)) void k_z_c2r(const cpx<TW> *__restrict__ in,
+                                                               double *__restrict__ out, long long lines, int hp,
+                                                               TW scale, const cpx<TW> *__restrict__ tw) {
     using P = PlanS<L>;
     using PL = typename P::radices;
     constexpr int E = P::E;
-    constexpr int T = ZGeom<L, E>::T, NT = ZGeom<L, E>::NT, LP = L;
+    constexpr int T = ZSGeom<L, TW>::T, NT = ZSGeom<L, TW>::NT;
     constexpr int HIN = L / 2 + 1;
-    __shared__ cpx<TW> lds[T * LP];
+    __shared__ cpx<TW> lds[L * T];
     const int tid = threadIdx.x;
-    const long long pair0 = (long long)blockIdx.x * T;
+    const long long s = (long long)blockIdx.x * T + tid % T;
+    const long long la = 2 * s, lb = la + 1;
     cpx<TW> v[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-        int s, p;
-        reg_pos_in<L, PL::first, T, NT, LP>(tid, e, s, p);
-        const long long la = 2 * (pair0 + s), lb = la + 1;
+        const int p = ci_pos_in<L, PL::first, T, NT>(tid, e);
         v[e] = {TW(0), TW(0)};
         if (la < lines) {
-            // the partner line beyond the end reads as zero
-            const cpx<TW> *fb = lb < lines ? in : nullptr;
             const cpx<TW> za = z_pair_at<L, HIN>(in, (const cpx<TW> *)nullptr, la * hp, p);
-            const cpx<TW> zb = fb ? z_pair_at<L, HIN>(in, (const cpx<TW> *)nullptr, lb * hp, p) : cpx<TW>{TW(0), TW(0)};
-            // za = X_a (Hermitian-extended), zb = X_b: Z = za + i zb
-            v[e] = {za.re - zb.im, za.im + zb.re};
+            const cpx<TW> zb =
+                lb < lines ? z_pair_at<L, HIN>(in, (const cpx<TW> *)nullptr, lb * hp, p) : cpx<TW>{TW(0), TW(0)};
+            v[e] = {za.re - zb.im, za.im + zb.re};  // X_a + i X_b
         }
     }
-    fft_reg<L, +1, T, NT, 1, LP, PL>(v, lds, tw, tid);
+    fft_reg<L, +1, T, NT, T, 1, PL>(v, lds, tw, tid);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-        int s, p;
-        reg_pos_out<L, PL::last, T, NT, LP>(tid, e, s, p);
-        const long long la = 2 * (pair0 + s), lb = la + 1;
+        const int p = ci_pos_out<L, PL::last, T, NT>(tid, e);
         if (la < lines) out[la * L + p] = double(v[e].re * scale);
         if (lb < lines) out[lb * L + p] = double(v[e].im * scale);
     }
 }
 
 template <int L, typename TW>
-__global__ __launch_bounds__(128) void k_z_r2c(const double *__restrict__ in, cpx<TW> *__restrict__ out,
-                                              long long lines, int hp, const cpx<TW> *__restrict__ tw) {
+__global__ __launch_bounds__((ZSGeom<L, TW>::NT)) void k_z_r2c(const double *__restrict__ in,
+                                                               cpx<TW> *__restrict__ out, long long lines, int hp,
+                                                               const cpx<TW> *__restrict__ tw) {
     using P = PlanS<L>;
     using PL = typename P::radices;
     constexpr int E = P::E;
-    constexpr int T = ZGeom<L, E>::T, NT = ZGeom<L, E>::NT, LP = L;
+    constexpr int T = ZSGeom<L, TW>::T, NT = ZSGeom<L, TW>::NT;
     constexpr int HOUT = L / 2 + 1;
-    __shared__ cpx<TW> lds[T * LP];
+    __shared__ cpx<TW> lds[L * T];
     const int tid = threadIdx.x;
     const long long pair0 = (long long)blockIdx.x * T;
+    const int t = tid % T;
+    const long long la = 2 * (pair0 + t), lb = la + 1;
     cpx<TW> v[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-        int s, p;
-        reg_pos_in<L, PL::first, T, NT, LP>(tid, e, s, p);
-        const long long la = 2 * (pair0 + s), lb = la + 1;
+        const int p = ci_pos_in<L, PL::first, T, NT>(tid, e);
         v[e] = {la < lines ? TW(in[la * L + p]) : TW(0), lb < lines ? TW(in[lb * L + p]) : TW(0)};
     }
-    fft_reg<L, -1, T, NT, 1, LP, PL>(v, lds, tw, tid);
+    fft_reg<L, -1, T, NT, T, 1, PL>(v, lds, tw, tid);
     __syncthreads();
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-        int s, p;
-        reg_pos_out<L, PL::last, T, NT, LP>(tid, e, s, p);
-        lds[s * LP + p] = v[e];
-    }
+    for (int e = 0; e < E; ++e) lds[ci_pos_out<L, PL::last, T, NT>(tid, e) * T + t] = v[e];
     __syncthreads();
     const TW h = TW(0.5);
     for (int e = tid; e < T * HOUT; e += NT) {
-        const int s = e / HOUT, k = e % HOUT;
-        const long long la = 2 * (pair0 + s), lb = la + 1;
-        const cpx<TW> zk = lds[s * LP + k], zm = lds[s * LP + (k ? L - k : 0)];
-        if (la < lines) out[la * hp + k] = {(zk.re + zm.re) * h, (zk.im - zm.im) * h};
-        if (lb < lines) out[lb * hp + k] = {(zk.im + zm.im) * h, (zm.re - zk.re) * h};
+        const int tt = e % T, k = e / T;
+        const long long a = 2 * (pair0 + tt), b = a + 1;
+        const cpx<TW> zk = lds[k * T + tt], zm = lds[(k ? L - k : 0) * T + tt];
+        if (a < lines) out[a * hp + k] = {(zk.re + zm.re) * h, (zk.im - zm.im) * h};
+        if (b < lines) out[b * hp + k] = {(zk.im + zm.im) * h, (zm.re - zk.re) * h};
     }
 }
 
@@ -656,12 +663,12 @@ int launch_z_nl_L(Prec wp, const void *in, void *out, long long lines, int hp, l
                   double s_in, double s_out, const void *tw, hipStream_t st) {
     if constexpr (L % 3 == 0 && (2 * L / 3) % 2 == 0) {
         if (wp == F64) {
-            using G = ZGeom<L, ZNLPlan<L, double>::type::E>;
+            using G = ZNLGeom<L, double>;
             hipLaunchKernelGGL((k_z_nl<L, double>), blocks_for<void>(lines, G::T), dim3(G::NT), 0, st,
                                reinterpret_cast<const cpx<double> *>(in), reinterpret_cast<cpx<double> *>(out), lines,
                                hp, fs_in, fs_out, s_in, s_out, reinterpret_cast<const cpx<double> *>(tw));
         } else {
-            using G = ZGeom<L, ZNLPlan<L, float>::type::E>;
+            using G = ZNLGeom<L, float>;
             hipLaunchKernelGGL((k_z_nl<L, float>), blocks_for<void>(lines, G::T), dim3(G::NT), 0, st,
                                reinterpret_cast<const cpx<float> *>(in), reinterpret_cast<cpx<float> *>(out), lines,
                                hp, fs_in, fs_out, float(s_in), float(s_out), reinterpret_cast<const cpx<float> *>(tw));
@@ -674,12 +681,13 @@ int launch_z_nl_L(Prec wp, const void *in, void *out, long long lines, int hp, l
 template <int L>
 int launch_z_max_L(Prec wp, const void *in, long long lines, int hp, long long fs, double s_in, const void *tw,
                    Scalars *sc, hipStream_t st) {
-    using G = ZGeom<L, PlanSmall<L>::type::E>;
     if (wp == F64) {
+        using G = ZMaxGeom<L, double>;
         hipLaunchKernelGGL((k_z_max<L, double>), blocks_for<void>(lines, G::T), dim3(G::NT), 0, st,
                            reinterpret_cast<const cpx<double> *>(in), lines, hp, fs, s_in,
                            reinterpret_cast<const cpx<double> *>(tw), sc);
     } else {
+        using G = ZMaxGeom<L, float>;
         hipLaunchKernelGGL((k_z_max<L, float>), blocks_for<void>(lines, G::T), dim3(G::NT), 0, st,
                            reinterpret_cast<const cpx<float> *>(in), lines, hp, fs, float(s_in),
                            reinterpret_cast<const cpx<float> *>(tw), sc);
@@ -690,7 +698,7 @@ int launch_z_max_L(Prec wp, const void *in, long long lines, int hp, long long f
 template <int L>
 int launch_z_c2r_L(const void *in, double *out, long long lines, int hp, double scale, const void *tw,
                    hipStream_t st) {
-    using G = ZGeom<L, PlanS<L>::E>;
+    using G = ZSGeom<L, double>;
     hipLaunchKernelGGL((k_z_c2r<L, double>), blocks_for<void>((lines + 1) / 2, G::T), dim3(G::NT), 0, st,
                        reinterpret_cast<const cpx<double> *>(in), out, lines, hp, scale,
                        reinterpret_cast<const cpx<double> *>(tw));
@@ -699,7 +707,7 @@ int launch_z_c2r_L(const void *in, double *out, long long lines, int hp, double 
 
 template <int L>
 int launch_z_r2c_L(const double *in, void *out, long long lines, int hp, const void *tw, hipStream_t st) {
-    using G = ZGeom<L, PlanS<L>::E>;
+    using G = ZSGeom<L, double>;
     hipLaunchKernelGGL((k_z_r2c<L, double>), blocks_for<void>((lines + 1) / 2, G::T), dim3(G::NT), 0, st, in,
                        reinterpret_cast<cpx<double> *>(out), lines, hp, reinterpret_cast<const cpx<double> *>(tw));
     return hipGetLastError() == hipSuccess ? 0 : 3;
