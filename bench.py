@@ -134,6 +134,75 @@ def run_gpu(args):
     return n, out, timing, fft, {"energy_rel_err": abs(emx - e64) / e64, "field_rel_l2": field_rel}
 
 
+KERNEL_OF_PASS = {  # kernel-name prefixes of the FP64 passes at the padded length M
+    "x_inverse": "k_inv<{M}, {N}, 2, double, double>",
+    "y_inverse": "k_inv<{M}, {N}, 4, double, double>",
+    "z_cross": "k_z_nl<{M}, double>",
+    "y_forward": "k_fwd<{M}, {N}, 0, 3, double>",
+    "x_forward": "k_fwd<{M}, {N}, 1, 3, double>",
+}
+
+
+def pmc_child(n: int) -> None:
+    """Workload of the counter runs: FP64 solver set-up (one state repack of
+    known size) and one adaptive step."""
+    from vonkarman_amd import GridSpec, IcType, Periodic3D
+    from vonkarman_amd import ic as vic
+
+    u0 = vic.von_karman_isotropic(n, seed=42)
+    s = Periodic3D(GridSpec.cubic(n), 1e-3, IcType.Spectral(u0))
+    s.steps(1)
+
+
+def measure_traffic(n: int, pass_name: str, timeout: float = 150.0):
+    """HBM bytes per launch of one pass from two rocprofv3 --pmc runs
+    (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch).  FETCH_SIZE is scaled by
+    the ratio of known bytes to the counter's reading for the state repack
+    kernel of the same run (a plain copy that reads 3 n^2 rows of n/2+1
+    complex doubles), WRITE_SIZE by the same ratio for its write (3 n^2 rows
+    of the padded row pitch).  Returns None if the profiler is not
+    available."""
+    import re
+    import shutil
+    import sqlite3
+    import subprocess
+    import tempfile
+
+    if not shutil.which("rocprofv3"):
+        return None
+    M, N, H = 3 * n // 2, n, n // 2 + 1
+    want = KERNEL_OF_PASS[pass_name].format(M=M, N=N)
+    known_read_kib = 3 * n * n * H * 16 / 1024.0
+    known_write_kib = 3 * n * n * ((H + 7) // 8 * 8) * 16 / 1024.0
+    vals = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="vk_pmc_", dir=os.environ.get("TMPDIR", "/tmp"))
+        cmd = ["rocprofv3", "--pmc", counter, "-d", d, "-o", "run", "--", sys.executable,
+               os.path.abspath(__file__), "--pmc-child", "--n", str(n)]
+        try:
+            subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=timeout, check=False)
+        except subprocess.TimeoutExpired:
+            return None
+        dbs = [os.path.join(r, f) for r, _, fs in os.walk(d) for f in fs if f.endswith("_results.db")]
+        if not dbs:
+            return None
+        con = sqlite3.connect(dbs[0])
+        rows = list(con.execute("select kernel_name, value from counters_collection order by dispatch_id"))
+        short = [(re.sub(r"\(.*", "", k).replace("void ", "").replace("vk::", ""), v) for k, v in rows]
+        rep = [v for k, v in short if k == "k_repack"]
+        ker = [v for k, v in short if k.startswith(want)]
+        shutil.rmtree(d, ignore_errors=True)
+        if not rep or not ker:
+            return None
+        vals[counter] = (rep[0], sum(ker) / len(ker))
+    fetch_scale = known_read_kib / vals["FETCH_SIZE"][0]
+    write_scale = known_write_kib / vals["WRITE_SIZE"][0]
+    read_b = vals["FETCH_SIZE"][1] * fetch_scale * 1024.0
+    write_b = vals["WRITE_SIZE"][1] * write_scale * 1024.0
+    return {"bytes": read_b + write_b, "read_bytes": read_b, "write_bytes": write_b,
+            "fetch_scale": fetch_scale, "write_scale": write_scale, "kernel": want}
+
+
 def cpu_baseline(n: int) -> dict:
     """The oracle (C++ f64 restatement of the reference's CPU path) on the
     host: one compute_nonlinear at n^3, all host threads; steps/s estimated as
@@ -162,7 +231,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true")
+    ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.pmc_child:
+        pmc_child(args.n)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU slab decomposition is a design note only)", file=sys.stderr)
@@ -176,6 +250,7 @@ def main():
     avg = timing["f64"]["avg_ms"]
     dom = max(avg, key=lambda k: avg[k])
     achieved = pb[dom] / (avg[dom] * 1e-3) / 1e9
+    traffic = None if args.no_traffic else measure_traffic(n, dom)
     cpu = None if args.no_cpu else cpu_baseline(n)
     sb = step_bytes(n, 8)
     line = {
@@ -195,8 +270,9 @@ def main():
                                "adaptive CFL 0.5, E(k) every 10 steps", "grid": n, "padded": 3 * n // 2,
                    "nu": 1e-3, "cfl_safety": 0.5, "spectrum_interval": 10},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_per_launch": pb[dom], "avg_launch_ms": avg[dom]},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None if traffic is None else traffic["bytes"],
+                     "bytes_per_launch": pb[dom], "avg_launch_ms": avg[dom],
+                     "traffic_detail": traffic},
         "cpu_baseline": cpu,
         "step_hbm_frac": sb * sps / (HBM_PEAK_GBS * 1e9),
         "step_bytes": sb,
