@@ -155,6 +155,34 @@ int vk_solver_physical(vk_solver *s, double *u_phys, double *w_phys);
 /* Device memory held by the solver, in bytes. */
 int vk_solver_device_bytes(const vk_solver *s, uint64_t *bytes);
 
+/* ------------------------------------------------------------------------ */
+/* Discrete adjoint (no counterpart in the reference).  Cotangents and       */
+/* gradients use the real inner product over the stored entries,            */
+/* <a,b> = sum Re(a)Re(b) + Im(a)Im(b) -- for a real objective J, the        */
+/* gradient array holds dJ/dRe + i dJ/dIm (the PyTorch convention).         */
+/* ------------------------------------------------------------------------ */
+#define VK_OBJ_NONE 0            /* only the given lam_T */
+#define VK_OBJ_FINAL_ENSTROPHY 1 /* J = enstrophy(u_T) (solver.rs:436-465) */
+#define VK_OBJ_DISSIPATION 2     /* J = sum_{n=1..T} dt 2 nu enstrophy(u_n) */
+
+/* g = (d compute_nonlinear / d u)^T lam at uhat (host arrays, state layout). */
+int vk_solver_nonlinear_vjp(vk_solver *s, const double *uhat, const double *lam, double *g);
+/* lam_in = (d step / d u)^T lam_out at the current state and dt; the state is
+ * not advanced.  Host / device pointers. */
+int vk_solver_step_vjp(vk_solver *s, const double *lam_out, double *lam_in);
+int vk_solver_step_vjp_device(vk_solver *s, const double *d_lam_out, double *d_lam_in);
+/* Gradient of J(u_T) + objective with respect to the current state u_0 over
+ * nsteps fixed-dt steps: lam_T (may be NULL) seeds dJ/du_T, `objective` adds
+ * the built-in terms.  Binomial checkpointing with at most max_snapshots
+ * stored states (each 3 n^2 (n/2+1) complex doubles); stage states are
+ * recomputed inside each step.  Writes J (built-in part), the gradient
+ * (same layout as the state) and the number of forward steps taken.  The
+ * solver state and clock are left at u_0. */
+int vk_solver_adjoint(vk_solver *s, int nsteps, int objective, const double *lam_T, int max_snapshots,
+                      double *J, double *grad, int64_t *forward_steps);
+int vk_solver_adjoint_device(vk_solver *s, int nsteps, int objective, const double *d_lam_T,
+                             int max_snapshots, double *J, double *d_grad, int64_t *forward_steps);
+
 /* Host-side Taylor-Green initial velocity (vonkarman-periodic/src/ic/
  * taylor_green.rs:10-38): real[3][n][n][n]. */
 int vk_ic_taylor_green(int n, double l, double *u_phys);

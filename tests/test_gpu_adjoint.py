@@ -1,0 +1,208 @@
+"""GPU discrete adjoint: VJPs against the oracle's adjoint (itself pinned by
+dot-product and finite-difference tests), checkpoint-schedule invariance,
+finite differences of trajectory objectives, and the torch.autograd wrapper."""
+import math
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+from oracle import oracle as O  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def ip(a, b):
+    return float(np.vdot(a, b).real)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def crandn(rng, shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+def hermitian_field(n, seed, amp=0.3):
+    """A smooth divergence-free field that is the R2C of a real field."""
+    rng = np.random.default_rng(seed)
+    x = np.arange(n) * 2 * math.pi / n
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    f = np.zeros((3, n, n, n))
+    for c in range(3):
+        for _ in range(4):
+            k = rng.integers(-3, 4, size=3)
+            ph = rng.uniform(0, 2 * math.pi)
+            f[c] += amp * rng.standard_normal() * np.cos(k[0] * X + k[1] * Y + k[2] * Z + ph)
+    uh = np.stack([O.r2c_3d(f[c]) for c in range(3)])
+    return O.leray(uh)
+
+
+def enstrophy_grad(u):
+    """dJ/du for J = enstrophy = sum w |curl u|^2 / n^6: 2 w curl(curl u) / n^6."""
+    n = u.shape[1]
+    w = np.full(n // 2 + 1, 2.0)
+    w[0] = w[n // 2] = 1.0
+    return 2.0 * w * O.curl(O.curl(u)) / float(n) ** 6
+
+
+@pytest.mark.parametrize("n", [8, 16, 32])
+def test_nonlinear_vjp_matches_oracle(hip_device, n):
+    from vonkarman_amd import GridSpec, Periodic3D
+
+    print(f"case nonlinear_vjp n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    s = (3, n, n, n // 2 + 1)
+    u, lam = crandn(rng, s), crandn(rng, s)
+    sol = Periodic3D(GridSpec.cubic(n), 0.01)
+    got = sol.nonlinear_vjp(u, lam)
+    assert rel(got, O.nonlinear_vjp(u, lam)) < 1e-12
+
+
+@pytest.mark.parametrize("n", [8, 16])
+def test_step_vjp_matches_oracle(hip_device, n):
+    from vonkarman_amd import GridSpec, IcType, Periodic3D
+
+    print(f"case step_vjp n={n}", flush=True)
+    nu, dt = 0.02, 5e-3
+    u = hermitian_field(n, 3)
+    lam = crandn(np.random.default_rng(5), u.shape)
+    sol = Periodic3D(GridSpec.cubic(n), nu, IcType.Spectral(u), init_dt=False)
+    sol.set_fixed_dt(dt)
+    got = sol.step_vjp(lam)
+    r = O.Solver(n, nu)
+    r.set_state(u)
+    r.set_dt(dt, True)
+    assert rel(got, r.step_vjp(lam)) < 1e-12
+    assert rel(sol.u_hat(), u) == 0.0  # the VJP leaves the state
+
+
+def _gpu_solver(n, nu, dt, u):
+    from vonkarman_amd import GridSpec, IcType, Periodic3D
+
+    s = Periodic3D(GridSpec.cubic(n), nu, IcType.Spectral(u), init_dt=False)
+    s.set_fixed_dt(dt)
+    return s
+
+
+def test_checkpoint_schedules_give_identical_gradients(hip_device):
+    n, nu, dt, T = 16, 0.01, 4e-3, 13
+    print("case adjoint schedules", flush=True)
+    u = hermitian_field(n, 7)
+    s = _gpu_solver(n, nu, dt, u)
+    res = [s.adjoint(T, "final_enstrophy", max_snapshots=k) for k in (1, 2, 4, T)]
+    for J, g, _ in res[1:]:
+        assert J == res[0][0]
+        assert np.array_equal(g, res[0][1])
+    # fewer snapshots, more recomputation; all T states visited
+    steps = [fs for _, _, fs in res]
+    assert steps[0] >= steps[1] >= steps[2] >= steps[3] >= T
+    assert np.array_equal(s.u_hat(), u) and s.step_count() == 0
+
+
+def test_adjoint_matches_oracle_vjp_chain(hip_device):
+    n, nu, dt, T = 16, 0.01, 4e-3, 6
+    print("case adjoint vs oracle chain", flush=True)
+    u0 = hermitian_field(n, 11)
+    s = _gpu_solver(n, nu, dt, u0)
+    J, g, _ = s.adjoint(T, "final_enstrophy", max_snapshots=3)
+    r = O.Solver(n, nu)
+    r.set_state(u0)
+    r.set_dt(dt, True)
+    states = [u0]
+    for _ in range(T):
+        r.step(1)
+        states.append(r.state())
+    assert abs(J - r.enstrophy()) < 1e-12 * r.enstrophy()
+    lam = enstrophy_grad(states[T])
+    for k in range(T - 1, -1, -1):
+        r.set_state(states[k])
+        lam = r.step_vjp(lam)
+    assert rel(g, lam) < 1e-11
+
+
+@pytest.mark.parametrize("objective", ["final_enstrophy", "dissipation"])
+def test_adjoint_gradient_matches_central_differences(hip_device, objective):
+    """Central differences (eps relative 1e-6) along seeded divergence-free
+    directions agree with <grad, d> to 1e-6 relative."""
+    n, nu, dt, T = 16, 0.01, 4e-3, 5
+    print(f"case adjoint fd {objective}", flush=True)
+    u0 = hermitian_field(n, 13)
+    s = _gpu_solver(n, nu, dt, u0)
+    J0, g, _ = s.adjoint(T, objective, max_snapshots=2)
+
+    def J_of(u):
+        t = _gpu_solver(n, nu, dt, u)
+        return t.adjoint(T, objective, max_snapshots=T)[0]
+
+    for seed in range(3):
+        d = hermitian_field(n, 100 + seed)
+        eps = 1e-6 * np.linalg.norm(u0) / np.linalg.norm(d)
+        fd = (J_of(u0 + eps * d) - J_of(u0 - eps * d)) / (2 * eps)
+        assert abs(fd - ip(g, d)) <= 1e-6 * abs(fd), (seed, fd, ip(g, d))
+
+
+def _torch_case():
+    """Trajectory/Objective autograd against the oracle's VJP chain.  Runs in
+    a process that imports torch before the library is loaded, so torch and the
+    library share one HIP runtime."""
+    import torch
+
+    assert torch.cuda.is_available(), "torch sees no GPU"
+    from vonkarman_amd._lib import hip_runtimes
+    from vonkarman_amd.autograd import objective, trajectory
+
+    n, nu, dt, T = 8, 0.02, 5e-3, 4
+    u0 = hermitian_field(n, 17)
+    s = _gpu_solver(n, nu, dt, u0)
+    assert len(hip_runtimes()) == 1, hip_runtimes()
+    t0 = torch.tensor(u0, dtype=torch.complex128, device="cuda", requires_grad=True)
+    uT = trajectory(t0, s, T, 2)
+    w = torch.tensor(crandn(np.random.default_rng(3), u0.shape), dtype=torch.complex128, device="cuda")
+    loss = (w.conj() * uT).real.sum()
+    loss.backward()
+    # reference: the oracle's VJP chain for lam_T = w (torch's complex gradient
+    # convention is dL/dRe + i dL/dIm, the real inner product used here)
+    r = O.Solver(n, nu)
+    r.set_state(u0)
+    r.set_dt(dt, True)
+    states = [u0]
+    for _ in range(T):
+        r.step(1)
+        states.append(r.state())
+    assert rel(uT.detach().cpu().numpy(), states[T]) < 1e-12
+    lam = w.cpu().numpy()
+    for k in range(T - 1, -1, -1):
+        r.set_state(states[k])
+        lam = r.step_vjp(lam)
+    assert rel(t0.grad.cpu().numpy(), lam) < 1e-11
+    # built-in objective through autograd, scaled upstream gradient
+    t1 = torch.tensor(u0, dtype=torch.complex128, device="cuda", requires_grad=True)
+    J = objective(t1, s, T, "final_enstrophy", 2)
+    (3.0 * J).backward()
+    Jn, gn, _ = s.adjoint(T, "final_enstrophy", max_snapshots=2)
+    assert abs(J.item() - Jn) <= 1e-15 * abs(Jn)
+    assert rel(t1.grad.cpu().numpy(), 3.0 * gn) < 1e-15
+    print("torch case ok", flush=True)
+
+
+def test_torch_autograd_trajectory(hip_device):
+    pytest.importorskip("torch")
+    print("case torch autograd (child process)", flush=True)
+    r = subprocess.run([sys.executable, "-u", os.path.abspath(__file__), "torch-case"], capture_output=True,
+                       text=True, timeout=600, cwd=ROOT)
+    print(r.stdout[-4000:], r.stderr[-4000:], flush=True)
+    assert r.returncode == 0 and "torch case ok" in r.stdout
+
+
+if __name__ == "__main__" and sys.argv[1:] == ["torch-case"]:
+    import torch  # noqa: F401  (first: the library then binds to torch's HIP runtime)
+
+    _torch_case()

@@ -20,6 +20,9 @@ VK_ERR_HIP = 3
 VK_ERR_NOMEM = 4
 VK_PREC_F64 = 0
 VK_PREC_MIXED = 1
+VK_OBJ_NONE = 0
+VK_OBJ_FINAL_ENSTROPHY = 1
+VK_OBJ_DISSIPATION = 2
 
 # Every symbol the header declares, with (restype, argtypes).
 _P = ctypes.c_void_p
@@ -104,6 +107,11 @@ SIGNATURES = {
     "vk_solver_physical": (_I, [_P, _P, _P]),
     "vk_solver_device_bytes": (_I, [_P, _PU64]),
     "vk_ic_taylor_green": (_I, [_I, _D, _P]),
+    "vk_solver_nonlinear_vjp": (_I, [_P, _P, _P, _P]),
+    "vk_solver_step_vjp": (_I, [_P, _P, _P]),
+    "vk_solver_step_vjp_device": (_I, [_P, _P, _P]),
+    "vk_solver_adjoint": (_I, [_P, _I, _I, _P, _I, _PD, _P, ctypes.POINTER(ctypes.c_int64)]),
+    "vk_solver_adjoint_device": (_I, [_P, _I, _I, _P, _I, _PD, _P, ctypes.POINTER(ctypes.c_int64)]),
 }
 
 _lib = None
@@ -132,6 +140,25 @@ def lib():
             f.argtypes = args
         _lib = L
     return _lib
+
+
+def hip_runtimes() -> set:
+    """Paths of the HIP runtimes (libamdhip64) mapped into this process.
+
+    torch's ROCm wheel carries its own libamdhip64 with the same soname as
+    /opt/rocm's.  Loaded after torch, this library binds to torch's copy (one
+    runtime, device pointers shared); loaded before torch, the process holds
+    two runtimes and torch cannot use the GPU."""
+    found = set()
+    try:
+        with open("/proc/self/maps") as f:
+            for line in f:
+                path = line.split()[-1] if len(line.split()) >= 6 else ""
+                if os.path.basename(path).startswith("libamdhip64.so"):
+                    found.add(os.path.realpath(path))
+    except OSError:
+        pass
+    return found
 
 
 def check(code: int, where: str):

@@ -310,6 +310,13 @@ struct vk_solver {
     DevBuf staging;         // unpadded [3][n][n][h] complex
     size_t total = 0;
     hipEvent_t ev[8] = {};
+    // discrete adjoint (allocated on first use)
+    bool vjp_ready = false;
+    DevBuf sa_, sb_, sc_, lam, gu, g1p, cot;  // stage states a, b, c and cotangents
+    DevBuf WA, WB;                            // VJP work arrays
+    std::vector<DevBuf> snaps;                 // checkpointed states
+    std::vector<int> split_tab;                // binomial schedule: best first advance per (steps, free slots)
+    int split_T = 0, split_S = 0;
     Scalars *dsc() { return static_cast<Scalars *>(sc.p); }
     size_t wbytes() const { return wp == F64 ? 16 : 8; }
 };
@@ -503,8 +510,10 @@ int vk_solver_destroy(vk_solver *s) {
     if (!s) return VK_OK;
     if (s->st) (void)hipStreamSynchronize(s->st);
     for (DevBuf *b : {&s->u, &s->x, &s->n1, &s->sacc, &s->A, &s->B, &s->tab, &s->sc, &s->tw_n, &s->tw_m,
-                      &s->partials, &s->spec, &s->staging})
+                      &s->partials, &s->spec, &s->staging, &s->sa_, &s->sb_, &s->sc_, &s->lam, &s->gu, &s->g1p,
+                      &s->cot, &s->WA, &s->WB})
         dev_free(*b);
+    for (DevBuf &b : s->snaps) dev_free(b);
     for (auto &e : s->ev)
         if (e) (void)hipEventDestroy(e);
     if (s->st) (void)hipStreamDestroy(s->st);
@@ -740,6 +749,315 @@ int vk_solver_physical(vk_solver *s, double *u_phys, double *w_phys) {
     std::string msg = g_err;
     vk_fft_plan_destroy(p);
     g_err = msg;
+    return r;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Discrete adjoint (no counterpart in the reference): VJP of the nonlinear
+// term and of one ETD-RK4 step, and the gradient of a trajectory objective
+// with binomial checkpointing.
+// ===========================================================================
+namespace {
+
+void *woff(void *base, long long elems, size_t elem_bytes) {
+    return static_cast<char *>(base) + (size_t)elems * elem_bytes;
+}
+
+int ensure_vjp(vk_solver *s) {
+    if (s->vjp_ready) return 0;
+    const long long N = s->n, M = s->m, Hp = s->hp;
+    const size_t state_bytes = (size_t)s->S * 3 * 16;
+    const size_t wa = (size_t)std::max(8 * N * M * Hp, 6 * M * M * Hp), wb = (size_t)9 * M * M * Hp;
+    int r = 0;
+    if ((r = dev_alloc(s->sa_, state_bytes, s->total)) || (r = dev_alloc(s->sb_, state_bytes, s->total)) ||
+        (r = dev_alloc(s->sc_, state_bytes, s->total)) || (r = dev_alloc(s->lam, state_bytes, s->total)) ||
+        (r = dev_alloc(s->gu, state_bytes, s->total)) || (r = dev_alloc(s->g1p, state_bytes, s->total)) ||
+        (r = dev_alloc(s->cot, state_bytes, s->total)) || (r = dev_alloc(s->WA, wa * s->wbytes(), s->total)) ||
+        (r = dev_alloc(s->WB, wb * s->wbytes(), s->total)))
+        return r;
+    VK_HIP(hipMemsetAsync(s->lam.p, 0, state_bytes, s->st));
+    s->vjp_ready = true;
+    return 0;
+}
+
+StageArgs adj_args(vk_solver *s, int stage, void *out) {
+    StageArgs a = stage_args(s, stage, out);
+    a.lam = s->lam.p;
+    a.gu = s->gu.p;
+    a.g1p = s->g1p.p;
+    a.cot = s->cot.p;
+    const double nt = (double)s->n * s->n * s->n;
+    a.inv_scale = 1.0 / ((double)s->m * s->m * s->m / nt);
+    return a;
+}
+
+// t = N'(v)^T (cotangent prepared in s->cot as inv_scale * P(lam)), then the
+// reverse-stage epilogue sa.  v is a state-layout field.
+int vjp_nl(vk_solver *s, const void *v, StageArgs &sa) {
+    const int N = s->n, M = s->m, Hp = s->hp, H = s->h;
+    const long long NMH = (long long)N * M * Hp, MMH = (long long)M * M * Hp, NH = (long long)N * Hp;
+    const long long MH = (long long)M * Hp;
+    const size_t wb = s->wbytes();
+    const double nt = (double)N * N * N;
+    // x inverse: u branch (5 fields) and cotangent branch (3 fields) -> WA
+    Strided g1 = make_strided((int)NH, Hp, H, (int)NH, 0, NH, s->S, Hp, MH, Hp, NMH);
+    VK_TRY(launch_inv(INV_U3DX, M, N, F64, s->wp, v, s->WA.p, s->tw_m.p, g1, N, s->ks, 1.0, s->st));
+    VK_TRY(launch_inv(INV_U3, M, N, F64, s->wp, s->cot.p, woff(s->WA.p, 5 * NMH, wb), s->tw_m.p, g1, N, s->ks, 1.0,
+                      s->st));
+    // y inverse: u, w (6 fields) and g (3 fields) -> WB
+    Strided g2 = make_strided((int)MH, Hp, H, (int)MH, 0, MH, NMH, Hp, MH, Hp, MMH);
+    VK_TRY(launch_inv(INV_UOMEGA, M, N, s->wp, s->wp, s->WA.p, s->WB.p, s->tw_m.p, g2, N, s->ks, 1.0, s->st));
+    VK_TRY(launch_inv(INV_U3, M, N, s->wp, s->wp, woff(s->WA.p, 5 * NMH, wb), woff(s->WB.p, 6 * MMH, wb), s->tw_m.p,
+                      g2, N, s->ks, 1.0, s->st));
+    // z: GU = C2R^T(w x g), GW = C2R^T(g x u) -> WA (6 fields)
+    VK_TRY(launch_z_vjp(M, s->wp, s->WB.p, s->WA.p, (long long)M * M, Hp, MMH, MMH, 1.0 / nt, 1.0 / nt, s->tw_m.p,
+                        s->st));
+    // y forward + truncation of the 6 fields -> WB (y-major)
+    Strided g3 = make_strided((int)MH, Hp, H, Hp, MH, Hp, MMH, (int)MH, 0, MH, NMH);
+    VK_TRY(launch_fwd(FWD_PLAIN, M, N, 6, s->wp, s->WA.p, s->WB.p, s->tw_m.p, g3, N, s->ks, 1.0, nullptr, s->st));
+    // x forward + truncation, t = GU + curl GW, reverse-stage epilogue
+    Strided g4 = make_strided((int)NH, Hp, H, Hp, MH, Hp, NMH, (int)NH, 0, NH, s->S);
+    VK_TRY(launch_fwd(FWD_VJP, M, N, 6, s->wp, s->WB.p, nullptr, s->tw_m.p, g4, N, s->ks, 1.0, &sa, s->st));
+    return 0;
+}
+
+// s->lam <- (d step / d u)^T s->lam at the state s->u (fixed dt); u unchanged
+int step_vjp(vk_solver *s) {
+    if (int r = ensure_vjp(s)) return r;
+    // forward recompute of the stage inputs a, b, c (solver.rs:253-343)
+    StageArgs f1 = stage_args(s, ST_1, s->sa_.p);
+    if (int r = nl_eval(s, s->u.p, f1, nullptr, nullptr)) return r;
+    StageArgs f2 = stage_args(s, ST_2, s->sb_.p);
+    if (int r = nl_eval(s, s->sa_.p, f2, nullptr, nullptr)) return r;
+    StageArgs f3 = stage_args(s, ST_3, s->sc_.p);
+    if (int r = nl_eval(s, s->sb_.p, f3, nullptr, nullptr)) return r;
+    // reverse sweep
+    StageArgs tmp = adj_args(s, AS_4, nullptr);
+    VK_TRY(launch_adj_prep(static_cast<const double *>(s->lam.p), static_cast<double *>(s->cot.p), s->n, s->hp,
+                           s->S, s->ks, tmp.inv_scale, static_cast<const double *>(s->tab.p), s->qmax + 1,
+                           s->dsc(), 1, s->st));
+    StageArgs a4 = adj_args(s, AS_4, nullptr);
+    if (int r = vjp_nl(s, s->sc_.p, a4)) return r;
+    StageArgs a3 = adj_args(s, AS_3, nullptr);
+    if (int r = vjp_nl(s, s->sb_.p, a3)) return r;
+    StageArgs a2 = adj_args(s, AS_2, nullptr);
+    if (int r = vjp_nl(s, s->sa_.p, a2)) return r;
+    StageArgs a1 = adj_args(s, AS_1, s->lam.p);
+    if (int r = vjp_nl(s, s->u.p, a1)) return r;
+    return 0;
+}
+
+int upload_unpadded(vk_solver *s, const double *src, void *dst, bool device) {
+    const size_t bytes = (size_t)3 * s->n * s->n * s->h * 16;
+    VK_HIP(hipMemcpyAsync(s->staging.p, src, bytes, device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s->st));
+    VK_TRY(launch_repack(s->staging.p, dst, (long long)3 * s->n * s->n, s->h, s->hp, s->h, 1, s->st));
+    return 0;
+}
+
+// ---- trajectory gradient with binomial checkpointing -----------------------
+// cost(n, f): forward steps needed to reverse n steps from a stored state with
+// f free snapshot slots; cost(1, f) = 0, cost(n, 0) = n(n-1)/2,
+// cost(n, f) = min_m  m + cost(n-m, f-1) + cost(m, f).
+void build_schedule(vk_solver *s, int T, int S) {
+    if (s->split_T >= T && s->split_S >= S) return;
+    std::vector<long long> cost((size_t)(T + 1) * S, 0);
+    std::vector<int> split((size_t)(T + 1) * S, 0);
+    auto C = [&](int n, int f) -> long long & { return cost[(size_t)n * S + f]; };
+    for (int n = 1; n <= T; ++n) {
+        C(n, 0) = (long long)n * (n - 1) / 2;
+        split[(size_t)n * S] = n - 1;
+    }
+    for (int f = 1; f < S; ++f)
+        for (int n = 2; n <= T; ++n) {
+            long long best = -1;
+            int bm = 1;
+            for (int m = 1; m < n; ++m) {
+                const long long c = m + C(n - m, f - 1) + C(m, f);
+                if (best < 0 || c < best) {
+                    best = c;
+                    bm = m;
+                }
+            }
+            C(n, f) = best;
+            split[(size_t)n * S + f] = bm;
+        }
+    s->split_tab.swap(split);
+    s->split_T = T;
+    s->split_S = S;
+}
+
+struct Adjoint {
+    vk_solver *s;
+    int T, S, objective;
+    bool lamT_given;
+    double J = 0.0;
+    long long forward_steps = 0;
+    double nt2;
+    int copy_state(void *dst, const void *src) {
+        VK_HIP(hipMemcpyAsync(dst, src, (size_t)s->S * 3 * 16, hipMemcpyDeviceToDevice, s->st));
+        return 0;
+    }
+    int advance(int k) {
+        for (int i = 0; i < k; ++i) {
+            if (int r = one_step(s, nullptr)) return r;
+            ++forward_steps;
+        }
+        return 0;
+    }
+    double enstrophy() {
+        if (launch_diag(static_cast<const double *>(s->u.p), s->n, s->hp, s->S, s->ks, s->nshells,
+                        static_cast<double *>(s->partials.p), s->nblocks_diag, nullptr, s->dsc(), s->st))
+            return NAN;
+        Scalars h;
+        if (read_scalars(s, h)) return NAN;
+        return h.diag[1];
+    }
+    // objective terms at state n (held in s->u): J contribution and source on lam
+    int objective_at(int n, bool final_state) {
+        const double nu = s->nu;
+        double dt = 0.0;
+        {
+            Scalars h;
+            if (int r = read_scalars(s, h)) return r;
+            dt = h.dt;
+        }
+        if (objective == VK_OBJ_FINAL_ENSTROPHY) {
+            if (final_state) {
+                J += enstrophy();
+                if (!lamT_given)
+                    VK_TRY(launch_add_curlcurl(static_cast<double *>(s->lam.p), static_cast<const double *>(s->u.p),
+                                               s->n, s->hp, s->S, s->ks, 2.0 / nt2, s->st));
+            }
+        } else if (objective == VK_OBJ_DISSIPATION) {
+            if (n >= 1) {
+                J += dt * 2.0 * nu * enstrophy();
+                VK_TRY(launch_add_curlcurl(static_cast<double *>(s->lam.p), static_cast<const double *>(s->u.p), s->n,
+                                           s->hp, s->S, s->ks, 4.0 * nu * dt / nt2, s->st));
+            }
+        }
+        return 0;
+    }
+    // the state at lo is in snapshot slot `slot`; free slots are [first, first + nfree)
+    int rev(int lo, int n, int slot, int first, int nfree) {
+        if (n == 1 || nfree == 0) {
+            for (int i = n - 1; i >= 0; --i) {
+                if (int r = copy_state(s->u.p, s->snaps[slot].p)) return r;
+                if (int r = advance(i)) return r;
+                const int at = lo + i;
+                if (at == T - 1) {  // the step into u_T: form u_T, its objective terms, then back
+                    // sa_ is rewritten by every step VJP, so it is free here
+                    if (int r = copy_state(s->sa_.p, s->u.p)) return r;
+                    if (int r = advance(1)) return r;
+                    if (int r = objective_at(T, true)) return r;
+                    if (int r = copy_state(s->u.p, s->sa_.p)) return r;
+                }
+                if (int r = step_vjp(s)) return r;
+                if (int r = objective_at(at, false)) return r;
+            }
+            return 0;
+        }
+        const int m = s->split_tab[(size_t)n * s->split_S + nfree];
+        if (int r = copy_state(s->u.p, s->snaps[slot].p)) return r;
+        if (int r = advance(m)) return r;
+        if (int r = copy_state(s->snaps[first].p, s->u.p)) return r;
+        if (int r = rev(lo + m, n - m, first, first + 1, nfree - 1)) return r;
+        return rev(lo, m, slot, first, nfree);
+    }
+};
+
+int adjoint_run(vk_solver *s, int nsteps, int objective, const double *lamT, int max_snapshots, double *J,
+                double *grad, bool device, long long *forward_steps) {
+    if (int r = check(s)) return r;
+    if (nsteps < 1) return fail(VK_ERR_ARG, "adjoint: nsteps must be >= 1");
+    if (!s->fixed) return fail(VK_ERR_ARG, "adjoint: needs fixed dt (vk_solver_set_dt(..., fixed=1))");
+    if (objective != VK_OBJ_NONE && objective != VK_OBJ_FINAL_ENSTROPHY && objective != VK_OBJ_DISSIPATION)
+        return fail(VK_ERR_ARG, "adjoint: unknown objective");
+    if (objective == VK_OBJ_NONE && !lamT) return fail(VK_ERR_ARG, "adjoint: no objective and no lam_T");
+    const int S = std::max(1, std::min(max_snapshots, nsteps));
+    if (int r = ensure_vjp(s)) return r;
+    while ((int)s->snaps.size() < S) {
+        DevBuf b;
+        if (int r = dev_alloc(b, (size_t)s->S * 3 * 16, s->total)) return r;
+        s->snaps.push_back(b);
+    }
+    build_schedule(s, nsteps, S);
+    Scalars h0;
+    if (int r = read_scalars(s, h0)) return r;
+    VK_HIP(hipMemsetAsync(s->lam.p, 0, (size_t)s->S * 3 * 16, s->st));
+    if (lamT)
+        if (int r = upload_unpadded(s, lamT, s->lam.p, device)) return r;
+    Adjoint A;
+    A.s = s;
+    A.T = nsteps;
+    A.S = S;
+    A.objective = objective;
+    A.lamT_given = lamT != nullptr;
+    const double nt = (double)s->n * s->n * s->n;
+    A.nt2 = nt * nt;
+    if (int r = A.copy_state(s->snaps[0].p, s->u.p)) return r;
+    int r = A.rev(0, nsteps, 0, 1, S - 1);
+    if (!r) r = A.copy_state(s->u.p, s->snaps[0].p);  // the solver is left at u_0
+    if (r) return r;
+    VK_HIP(hipMemcpyAsync(s->sc.p, &h0, sizeof(Scalars), hipMemcpyHostToDevice, s->st));  // clock restored
+    if (grad) {
+        const size_t bytes = (size_t)3 * s->n * s->n * s->h * 16;
+        VK_TRY(launch_repack(s->lam.p, s->staging.p, (long long)3 * s->n * s->n, s->hp, s->h, s->h, 1, s->st));
+        VK_HIP(hipMemcpyAsync(grad, s->staging.p, bytes, device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                              s->st));
+    }
+    VK_HIP(hipStreamSynchronize(s->st));
+    if (J) *J = A.J;
+    if (forward_steps) *forward_steps = A.forward_steps;
+    return VK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vk_solver_nonlinear_vjp(vk_solver *s, const double *uhat, const double *lam, double *g) {
+    if (int r = check(s)) return r;
+    if (!uhat || !lam || !g) return fail(VK_ERR_ARG, "null pointer");
+    if (int r = ensure_vjp(s)) return r;
+    if (int r = upload_unpadded(s, uhat, s->x.p, false)) return r;
+    if (int r = upload_unpadded(s, lam, s->lam.p, false)) return r;
+    StageArgs a = adj_args(s, AS_RAW, s->n1.p);
+    VK_TRY(launch_adj_prep(static_cast<const double *>(s->lam.p), static_cast<double *>(s->cot.p), s->n, s->hp, s->S,
+                           s->ks, a.inv_scale, static_cast<const double *>(s->tab.p), s->qmax + 1, s->dsc(), 0,
+                           s->st));
+    if (int r = vjp_nl(s, s->x.p, a)) return r;
+    return get_state_impl(s, g, false, s->n1.p);
+}
+
+static int step_vjp_impl(vk_solver *s, const double *lam_out, double *lam_in, bool device) {
+    if (int r = check(s)) return r;
+    if (!lam_out || !lam_in) return fail(VK_ERR_ARG, "null pointer");
+    if (int r = ensure_vjp(s)) return r;
+    if (int r = upload_unpadded(s, lam_out, s->lam.p, device)) return r;
+    if (int r = step_vjp(s)) return r;
+    return get_state_impl(s, lam_in, device, s->lam.p);
+}
+int vk_solver_step_vjp(vk_solver *s, const double *lam_out, double *lam_in) {
+    return step_vjp_impl(s, lam_out, lam_in, false);
+}
+int vk_solver_step_vjp_device(vk_solver *s, const double *lam_out, double *lam_in) {
+    return step_vjp_impl(s, lam_out, lam_in, true);
+}
+
+int vk_solver_adjoint(vk_solver *s, int nsteps, int objective, const double *lam_T, int max_snapshots, double *J,
+                      double *grad, int64_t *forward_steps) {
+    long long fs = 0;
+    int r = adjoint_run(s, nsteps, objective, lam_T, max_snapshots, J, grad, false, &fs);
+    if (forward_steps) *forward_steps = fs;
+    return r;
+}
+int vk_solver_adjoint_device(vk_solver *s, int nsteps, int objective, const double *d_lam_T, int max_snapshots,
+                             double *J, double *d_grad, int64_t *forward_steps) {
+    long long fs = 0;
+    int r = adjoint_run(s, nsteps, objective, d_lam_T, max_snapshots, J, d_grad, true, &fs);
+    if (forward_steps) *forward_steps = fs;
     return r;
 }
 

@@ -29,6 +29,13 @@ int launch_z_nl(int L, Prec wp, const void *in, void *out, long long lines, int 
     switch (L) { VK_FOR_EACH_LENGTH(VK_CASE_ZNL) }
     return 2;
 }
+#define VK_CASE_ZVJP(LEN) \
+    case LEN: return launch_z_vjp_L<LEN>(wp, in, out, lines, hp, fs_in, fs_out, s_u, s_o, tw, st);
+int launch_z_vjp(int L, Prec wp, const void *in, void *out, long long lines, int hp, long long fs_in,
+                 long long fs_out, double s_u, double s_o, const void *tw, hipStream_t st) {
+    switch (L) { VK_FOR_EACH_LENGTH(VK_CASE_ZVJP) }
+    return 2;
+}
 #define VK_CASE_ZMAX(LEN) \
     case LEN: return launch_z_max_L<LEN>(wp, in, lines, hp, fs, s_in, tw, sc, st);
 int launch_z_max(int L, Prec wp, const void *in, long long lines, int hp, long long fs, double s_in,
@@ -285,6 +292,95 @@ int launch_diag(const double *u, int n, int hp, long long fs, double ks, int nsh
     const double nt = (double)n * n * n;
     hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(256), 0, st, partials, nblocks, nshells, 1.0 / (nt * nt), spec_out,
                        sc);
+    return last_launch();
+}
+
+// ---------------------------------------------------------------------------
+// adjoint helpers (state layout [3][n][n][hp], complex double)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void mode_k(long long i, int n, int hp, double ks, int &ix, int &iy, int &iz, double &kx,
+                                       double &ky, double &kz) {
+    iz = (int)(i % hp);
+    const long long r = i / hp;
+    iy = (int)(r % n);
+    ix = (int)(r / n);
+    kx = double(wavenum(ix, n)) * ks;
+    ky = double(wavenum(iy, n)) * ks;
+    kz = double(iz) * ks;
+}
+
+// cot = inv_scale * P(coef * lam), coef = dt * b4 (table row 6) or 1
+__global__ void k_adj_prep(const cpx<double> *__restrict__ lam, cpx<double> *__restrict__ cot, int n, int hp,
+                           long long fs, double ks, double inv_scale, const double *__restrict__ tab, int ts,
+                           const Scalars *sc, int use_b4) {
+    const long long total = (long long)n * n * hp;
+    const int h = n / 2 + 1;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+         i += (long long)gridDim.x * blockDim.x) {
+        int ix, iy, iz;
+        double kx, ky, kz;
+        mode_k(i, n, hp, ks, ix, iy, iz, kx, ky, kz);
+        if (iz >= h) continue;
+        double coef = 1.0;
+        if (use_b4) {
+            const int wx = wavenum(ix, n), wy = wavenum(iy, n);
+            coef = sc->dt * tab[6 * ts + wx * wx + wy * wy + iz * iz];
+        }
+        cpx<double> x[3];
+        for (int f = 0; f < 3; ++f) x[f] = {coef * lam[f * fs + i].re, coef * lam[f * fs + i].im};
+        const double k2 = kx * kx + ky * ky + kz * kz;
+        if (!(k2 < 1e-30)) {
+            const double inv = 1.0 / k2;
+            const double dr = kx * x[0].re + ky * x[1].re + kz * x[2].re;
+            const double di = kx * x[0].im + ky * x[1].im + kz * x[2].im;
+            x[0] = {x[0].re - kx * dr * inv, x[0].im - kx * di * inv};
+            x[1] = {x[1].re - ky * dr * inv, x[1].im - ky * di * inv};
+            x[2] = {x[2].re - kz * dr * inv, x[2].im - kz * di * inv};
+        }
+        for (int f = 0; f < 3; ++f) cot[f * fs + i] = {x[f].re * inv_scale, x[f].im * inv_scale};
+    }
+}
+int launch_adj_prep(const double *lam, double *cot, int n, int hp, long long fs, double ks, double inv_scale,
+                    const double *tab, int tab_stride, const Scalars *sc, int use_b4, hipStream_t st) {
+    const long long total = (long long)n * n * hp;
+    const unsigned nb = (unsigned)std::min<long long>((total + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_adj_prep, dim3(nb), dim3(256), 0, st, reinterpret_cast<const cpx<double> *>(lam),
+                       reinterpret_cast<cpx<double> *>(cot), n, hp, fs, ks, inv_scale, tab, tab_stride, sc, use_b4);
+    return last_launch();
+}
+
+// lam += coef * w * curl(curl u) = coef * w * (k^2 u - k (k.u)), w the R2C weight
+__global__ void k_add_curlcurl(cpx<double> *__restrict__ lam, const cpx<double> *__restrict__ u, int n, int hp,
+                               long long fs, double ks, double coef) {
+    const long long total = (long long)n * n * hp;
+    const int h = n / 2 + 1;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+         i += (long long)gridDim.x * blockDim.x) {
+        int ix, iy, iz;
+        double kx, ky, kz;
+        mode_k(i, n, hp, ks, ix, iy, iz, kx, ky, kz);
+        if (iz >= h) continue;
+        const double w = (iz == 0 || iz == n / 2) ? 1.0 : 2.0;
+        const cpx<double> a = u[i], b = u[fs + i], c = u[2 * fs + i];
+        // curl(curl u) with the curl of spectral_ops.rs:93-104 applied twice
+        const cpx<double> w0 = {-(ky * c.im - kz * b.im), ky * c.re - kz * b.re};
+        const cpx<double> w1 = {-(kz * a.im - kx * c.im), kz * a.re - kx * c.re};
+        const cpx<double> w2 = {-(kx * b.im - ky * a.im), kx * b.re - ky * a.re};
+        const cpx<double> c0 = {-(ky * w2.im - kz * w1.im), ky * w2.re - kz * w1.re};
+        const cpx<double> c1 = {-(kz * w0.im - kx * w2.im), kz * w0.re - kx * w2.re};
+        const cpx<double> c2 = {-(kx * w1.im - ky * w0.im), kx * w1.re - ky * w0.re};
+        const double s = coef * w;
+        lam[i] = {lam[i].re + s * c0.re, lam[i].im + s * c0.im};
+        lam[fs + i] = {lam[fs + i].re + s * c1.re, lam[fs + i].im + s * c1.im};
+        lam[2 * fs + i] = {lam[2 * fs + i].re + s * c2.re, lam[2 * fs + i].im + s * c2.im};
+    }
+}
+int launch_add_curlcurl(double *lam, const double *u, int n, int hp, long long fs, double ks, double coef,
+                        hipStream_t st) {
+    const long long total = (long long)n * n * hp;
+    const unsigned nb = (unsigned)std::min<long long>((total + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_add_curlcurl, dim3(nb), dim3(256), 0, st, reinterpret_cast<cpx<double> *>(lam),
+                       reinterpret_cast<const cpx<double> *>(u), n, hp, fs, ks, coef);
     return last_launch();
 }
 

@@ -13,6 +13,8 @@ template int launch_fwd_L<VK_L>(int, int, int, Prec, const void *, void *, const
                                 double, double, const StageArgs *, hipStream_t);
 template int launch_z_nl_L<VK_L>(Prec, const void *, void *, long long, int, long long, long long, double, double,
                                  const void *, hipStream_t);
+template int launch_z_vjp_L<VK_L>(Prec, const void *, void *, long long, int, long long, long long, double, double,
+                                  const void *, hipStream_t);
 template int launch_z_max_L<VK_L>(Prec, const void *, long long, int, long long, double, const void *, Scalars *,
                                   hipStream_t);
 template int launch_z_c2r_L<VK_L>(const void *, double *, long long, int, double, const void *, hipStream_t);

@@ -213,12 +213,89 @@ __device__ __forceinline__ void stage_epilogue(const StageArgs &sa, long long id
     }
 }
 
+// Leray projection of one mode (vonkarman-core/src/spectral_ops.rs:114-151)
+__device__ __forceinline__ void leray_mode(double kx, double ky, double kz, cpx<double> x[3]) {
+    const double k2 = kx * kx + ky * ky + kz * kz;
+    if (k2 < 1e-30) return;
+    const double inv = 1.0 / k2;
+    const double dr = kx * x[0].re + ky * x[1].re + kz * x[2].re;
+    const double di = kx * x[0].im + ky * x[1].im + kz * x[2].im;
+    x[0] = {x[0].re - kx * dr * inv, x[0].im - kx * di * inv};
+    x[1] = {x[1].re - ky * dr * inv, x[1].im - ky * di * inv};
+    x[2] = {x[2].re - kz * dr * inv, x[2].im - kz * di * inv};
+}
+
+// Reverse sweep of etd_rk4_step (vonkarman-periodic/src/solver.rs:221-379):
+// with t = N'(stage state)^T (cotangent of that stage's N), the stage
+// cotangents propagate as
+//   AS_4 (stage c):  gu = e (lam + t);  g1p = dt (b1 lam - a41 t);  cot <- dt (b23 lam + 2 a41 t)
+//   AS_3 (stage b):  gu += e_half t;    cot <- dt (b23 lam + a31 t)
+//   AS_2 (stage a):  gu += e_half t;    cot <- g1p + dt a21 t
+//   AS_1 (stage u):  out = gu + t
+// where cot <- x stores inv_scale * P(x), the input the next VJP's inverse
+// passes expect.  All coefficient operators are real diagonal, hence self-adjoint.
+__device__ __forceinline__ void adj_epilogue(const StageArgs &sa, long long idx, int q, double kx, double ky,
+                                             double kz, const cpx<double> t[3]) {
+    const long long fs = sa.fs;
+    const cpx<double> *lam = reinterpret_cast<const cpx<double> *>(sa.lam);
+    cpx<double> *gu = reinterpret_cast<cpx<double> *>(sa.gu);
+    cpx<double> *g1p = reinterpret_cast<cpx<double> *>(sa.g1p);
+    cpx<double> *cot = reinterpret_cast<cpx<double> *>(sa.cot);
+    cpx<double> *out = reinterpret_cast<cpx<double> *>(sa.out);
+    const double *tab = sa.tab;
+    const int ts = sa.tab_stride;
+    const double dt = sa.sc->dt;
+    cpx<double> c[3];
+    if (sa.stage == AS_RAW) {
+        for (int f = 0; f < 3; ++f) out[f * fs + idx] = t[f];
+        return;
+    }
+    if (sa.stage == AS_1) {
+        for (int f = 0; f < 3; ++f) {
+            const cpx<double> g = gu[f * fs + idx];
+            out[f * fs + idx] = {g.re + t[f].re, g.im + t[f].im};
+        }
+        return;
+    }
+    if (sa.stage == AS_4) {
+        const double e = tab[0 * ts + q], a41 = tab[3 * ts + q], b1 = tab[4 * ts + q], b23 = tab[5 * ts + q];
+        for (int f = 0; f < 3; ++f) {
+            const cpx<double> l = lam[f * fs + idx];
+            gu[f * fs + idx] = {e * (l.re + t[f].re), e * (l.im + t[f].im)};
+            g1p[f * fs + idx] = {dt * (b1 * l.re - a41 * t[f].re), dt * (b1 * l.im - a41 * t[f].im)};
+            c[f] = {dt * (b23 * l.re + 2.0 * a41 * t[f].re), dt * (b23 * l.im + 2.0 * a41 * t[f].im)};
+        }
+    } else if (sa.stage == AS_3) {
+        const double eh = tab[1 * ts + q], a31 = tab[2 * ts + q], b23 = tab[5 * ts + q];
+        for (int f = 0; f < 3; ++f) {
+            const cpx<double> l = lam[f * fs + idx];
+            const cpx<double> g = gu[f * fs + idx];
+            gu[f * fs + idx] = {g.re + eh * t[f].re, g.im + eh * t[f].im};
+            c[f] = {dt * (b23 * l.re + a31 * t[f].re), dt * (b23 * l.im + a31 * t[f].im)};
+        }
+    } else {  // AS_2
+        const double eh = tab[1 * ts + q], a21 = tab[2 * ts + q];
+        for (int f = 0; f < 3; ++f) {
+            const cpx<double> g = gu[f * fs + idx];
+            const cpx<double> p = g1p[f * fs + idx];
+            gu[f * fs + idx] = {g.re + eh * t[f].re, g.im + eh * t[f].im};
+            c[f] = {p.re + dt * a21 * t[f].re, p.im + dt * a21 * t[f].im};
+        }
+    }
+    leray_mode(kx, ky, kz, c);
+    for (int f = 0; f < 3; ++f) cot[f * fs + idx] = {c[f].re * sa.inv_scale, c[f].im * sa.inv_scale};
+}
+
 template <int L, int MODE>
 struct FwdPlan {
     using type = PlanS<L>;
 };
 template <int L>
 struct FwdPlan<L, FWD_STAGE> {
+    using type = PlanE12<L>;
+};
+template <int L>
+struct FwdPlan<L, FWD_VJP> {
     using type = PlanE12<L>;
 };
 template <int L, int MODE, typename TW>
@@ -240,10 +317,11 @@ __global__ __launch_bounds__((FwdGeom<L, MODE, TW>::NT)) void k_fwd(const cpx<TW
     const cpx<TW> zero = {TW(0), TW(0)};
     cpx<TW> v[E];
     // kept outputs of the last radix under truncation: m in [0, RP/3) u [2RP/3, RP)
-    constexpr int KR = (MODE == FWD_STAGE) ? 2 * RP / 3 : 1;
-    constexpr int K = (MODE == FWD_STAGE) ? (E / RP) * KR : 1;
-    static_assert(MODE != FWD_STAGE || (RP % 3 == 0 && 3 * LOUT == 2 * L), "stage pass needs a 3/2-padded plan");
-    cpx<TW> r[MODE == FWD_STAGE ? 3 : 1][K];
+    constexpr bool KEEP = MODE == FWD_STAGE || MODE == FWD_VJP;
+    constexpr int KR = KEEP ? 2 * RP / 3 : 1;
+    constexpr int K = KEEP ? (E / RP) * KR : 1;
+    static_assert(!KEEP || (RP % 3 == 0 && 3 * LOUT == 2 * L), "stage pass needs a 3/2-padded plan");
+    cpx<TW> r[KEEP ? 3 : 1][K];
 
     const int col = col0 + tid % T;
     const bool colok = col < g.ncols && (col % g.hp) < g.hvalid;
@@ -257,7 +335,72 @@ __global__ __launch_bounds__((FwdGeom<L, MODE, TW>::NT)) void k_fwd(const cpx<TW
             v[e] = colok ? src[(long long)p * g.in_ls] : zero;
         }
     };
-    if constexpr (MODE == FWD_STAGE) {
+    // wavenumbers of kept element (b, mk)
+    auto kept_k = [&](int b, int mk, int &j, int &wx, int &wy, int &iz) {
+        const int m = mk < RP / 3 ? mk : mk + RP / 3;
+        const int p = ci_pos_out<L, RP, T, NT>(tid, b * RP + m);
+        j = unpad_index(p, LOUT, L);
+        const int iy = col / g.hp;
+        iz = col % g.hp;
+        wx = wavenum(j, n);
+        wy = wavenum(iy, n);
+    };
+    if constexpr (MODE == FWD_VJP) {
+        // t = GU + curl(GW): GW (fields 3..5) first, curled in registers, then GU added
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            load_field(3 + f);
+            fft_reg<L, -1, T, NT, T, 1, PL>(v, lds, tw, tid);
+#pragma unroll
+            for (int b = 0; b < E / RP; ++b)
+#pragma unroll
+                for (int mk = 0; mk < KR; ++mk) {
+                    const int m = mk < RP / 3 ? mk : mk + RP / 3;
+                    r[f][b * KR + mk] = cscale(v[b * RP + m], scale);
+                }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int b = 0; b < E / RP; ++b)
+#pragma unroll
+            for (int mk = 0; mk < KR; ++mk) {
+                int j, wx, wy, iz;
+                kept_k(b, mk, j, wx, wy, iz);
+                const TW kx = TW(double(wx) * ks), ky = TW(double(wy) * ks), kz = TW(double(iz) * ks);
+                const int i = b * KR + mk;
+                const cpx<TW> w0 = r[0][i], w1 = r[1][i], w2 = r[2][i];
+                // SpectralOps::curl (vonkarman-core/src/spectral_ops.rs:93-104)
+                r[0][i] = {-(ky * w2.im - kz * w1.im), ky * w2.re - kz * w1.re};
+                r[1][i] = {-(kz * w0.im - kx * w2.im), kz * w0.re - kx * w2.re};
+                r[2][i] = {-(kx * w1.im - ky * w0.im), kx * w1.re - ky * w0.re};
+            }
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            load_field(f);
+            fft_reg<L, -1, T, NT, T, 1, PL>(v, lds, tw, tid);
+#pragma unroll
+            for (int b = 0; b < E / RP; ++b)
+#pragma unroll
+                for (int mk = 0; mk < KR; ++mk) {
+                    const int m = mk < RP / 3 ? mk : mk + RP / 3;
+                    r[f][b * KR + mk] = cadd(r[f][b * KR + mk], cscale(v[b * RP + m], scale));
+                }
+            __syncthreads();
+        }
+        if (colok) {
+#pragma unroll
+            for (int b = 0; b < E / RP; ++b)
+#pragma unroll
+                for (int mk = 0; mk < KR; ++mk) {
+                    int j, wx, wy, iz;
+                    kept_k(b, mk, j, wx, wy, iz);
+                    const int i = b * KR + mk;
+                    const cpx<double> tt[3] = {cvt<double>(r[0][i]), cvt<double>(r[1][i]), cvt<double>(r[2][i])};
+                    adj_epilogue(sa, cout + (long long)j * g.out_ls, wx * wx + wy * wy + iz * iz,
+                                 double(wx) * ks, double(wy) * ks, double(iz) * ks, tt);
+                }
+        }
+    } else if constexpr (MODE == FWD_STAGE) {
         // three fields; the kept outputs of each stay in registers (static indices)
 #pragma unroll
         for (int f = 0; f < 3; ++f) {
@@ -436,6 +579,129 @@ __global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) void k_z_nl(const cpx<TW> *__
     }
 }
 
+// Pair spectrum with per-half DC factors (the z transpose of an R2C counts
+// its k=0 term once where the C2R extension would count it twice).
+template <int L, int HIN, typename TW>
+__device__ __forceinline__ cpx<TW> z_pair_dc(const cpx<TW> *__restrict__ fa, const cpx<TW> *__restrict__ fb,
+                                            long long base, int p, TW dca, TW dcb) {
+    const cpx<TW> z = z_pair_at<L, HIN>(fa, fb, base, p);
+    return p == 0 ? cpx<TW>{z.re * dca, z.im * dcb} : z;
+}
+
+// split the forward DFT of (a + i b) held in registers into the truncated
+// half spectra of a and b, times c_k s (c_0 = 1, c_k = 2 otherwise): the
+// transpose of the unnormalised C2R applied to a and b.
+template <int L, int T, int NT, typename RL, int E, int HOUT, typename TW>
+__device__ __forceinline__ void z_split_store(const cpx<TW> (&v)[E], cpx<TW> *lds, int tid, long long line0,
+                                              long long lines, int hp, cpx<TW> *oa, cpx<TW> *ob, TW s) {
+    const int t = tid % T;
+#pragma unroll
+    for (int e = 0; e < E; ++e) lds[ci_pos_out<L, RL::last, T, NT>(tid, e) * T + t] = v[e];
+    __syncthreads();
+    for (int e = tid; e < T * HOUT; e += NT) {
+        const int tt = e % T, k = e / T;
+        const long long line = line0 + tt;
+        const cpx<TW> zk = lds[k * T + tt], zm = lds[(k ? L - k : 0) * T + tt];
+        const TW h = k ? s : TW(0.5) * s;
+        if (line < lines) {
+            oa[line * hp + k] = {(zk.re + zm.re) * h, (zk.im - zm.im) * h};
+            ob[line * hp + k] = {(zk.im + zm.im) * h, (zm.re - zk.re) * h};
+        }
+    }
+    __syncthreads();
+}
+
+// VJP z pass.  in: fields 0..5 = u, w (padded half spectra, as the forward z
+// pass reads them) and 6..8 = g, the cotangent of the truncated R2C output.
+// out: 0..2 = GU = C2R^T (w x g), 3..5 = GW = C2R^T (g x u), truncated.
+template <int L, typename TW>
+__global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) void k_z_vjp(const cpx<TW> *__restrict__ in,
+                                                                cpx<TW> *__restrict__ out, long long lines, int hp,
+                                                                long long fs_in, long long fs_out, TW s_u, TW s_o,
+                                                                const cpx<TW> *__restrict__ tw) {
+    using P = ZNLP<L, TW>;
+    using PL = typename P::radices;
+    using RL = typename Reverse<PL>::type;
+    constexpr int E = P::E;
+    constexpr int T = ZNLGeom<L, TW>::T, NT = ZNLGeom<L, TW>::NT;
+    constexpr int HIN = L / 3 + 1, HOUT = HIN;
+    __shared__ cpx<TW> lds[L * T];
+    const int tid = threadIdx.x;
+    const long long line0 = (long long)blockIdx.x * T;
+    const int t = tid % T;
+    const bool ok = line0 + t < lines;
+    const long long base = (line0 + t) * hp;
+    const TW one = TW(1), two = TW(2), half = TW(0.5);
+    cpx<TW> v[E];
+    TW g0[E], g1[E], r0[E], r1[E], r2[E], r3[E];
+
+    auto inverse = [&](int fa, int fb, TW dca, TW dcb) {
+        const cpx<TW> *pa = opaque(in) + fa * fs_in, *pb = fb >= 0 ? in + fb * fs_in : nullptr;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int p = ci_pos_in<L, PL::first, T, NT>(tid, e);
+            v[e] = ok ? z_pair_dc<L, HIN>(pa, pb, base, p, dca, dcb) : cpx<TW>{TW(0), TW(0)};
+        }
+        fft_reg<L, +1, T, NT, T, 1, PL>(v, lds, tw, tid);
+    };
+    // Cotangent fields are paired only with cotangent fields: a complex FFT
+    // of (a + i b) leaks roundoff of the larger of a, b into the other, and u, w
+    // are many orders above g.
+    // g = R2C^T gin: C2R with the DC counted once, times 1/2
+    inverse(6, 7, two, two);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        g0[e] = v[e].re * half;
+        g1[e] = v[e].im * half;
+    }
+    __syncthreads();
+    inverse(8, -1, two, one);  // (gz, 0)
+#pragma unroll
+    for (int e = 0; e < E; ++e) r0[e] = v[e].re * half;  // g2
+    __syncthreads();
+    inverse(0, 1, one, one);  // (ux, uy): gW2 = g0 U1 - g1 U0 and the U0, U1 parts of gW0, gW1
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const TW u0 = v[e].re * s_u, u1 = v[e].im * s_u, gz = r0[e];
+        r1[e] = -gz * u1;                 // gW0 = g1 U2 - g2 U1
+        r2[e] = gz * u0;                  // gW1 = g2 U0 - g0 U2
+        r3[e] = g0[e] * u1 - g1[e] * u0;  // gW2
+    }
+    __syncthreads();
+    inverse(2, 3, one, one);  // (uz, wx): gW0, gW1 complete; the W0 parts of gU1, gU2
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const TW u2 = v[e].re * s_u, w0 = v[e].im * s_u, gz = r0[e];
+        const TW gw0 = r1[e] + g1[e] * u2, gw1 = r2[e] - g0[e] * u2;
+        r1[e] = -w0 * gz;     // gU1 = w2 g0 - w0 g2
+        r2[e] = w0 * g1[e];   // gU2 = w0 g1 - w1 g0
+        v[e] = {gw0, gw1};
+    }
+    __syncthreads();
+    fft_reg<L, -1, T, NT, T, 1, RL>(v, lds, tw, tid);
+    __syncthreads();
+    z_split_store<L, T, NT, RL, E, HOUT>(v, lds, tid, line0, lines, hp, out + 3 * fs_out, out + 4 * fs_out, s_o);
+    // live: g0, g1, g2 = r0, gU1 part = r1, gU2 part = r2, gW2 = r3
+    inverse(4, 5, one, one);  // (wy, wz): gU complete
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const TW w1 = v[e].re * s_u, w2 = v[e].im * s_u, gz = r0[e];
+        const TW gu0 = w1 * gz - w2 * g1[e];  // gU0 = w1 g2 - w2 g1
+        const TW gu1 = r1[e] + w2 * g0[e];
+        r0[e] = r2[e] - w1 * g0[e];           // gU2
+        v[e] = {gu0, gu1};
+    }
+    __syncthreads();
+    fft_reg<L, -1, T, NT, T, 1, RL>(v, lds, tw, tid);
+    __syncthreads();
+    z_split_store<L, T, NT, RL, E, HOUT>(v, lds, tid, line0, lines, hp, out + 0 * fs_out, out + 1 * fs_out, s_o);
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = {r0[e], r3[e]};
+    fft_reg<L, -1, T, NT, T, 1, RL>(v, lds, tw, tid);
+    __syncthreads();
+    z_split_store<L, T, NT, RL, E, HOUT>(v, lds, tid, line0, lines, hp, out + 2 * fs_out, out + 5 * fs_out, s_o);
+}
+
 // C2R of three unpadded fields and the maximum of |v|^2 (CFL speed or |w|).
 template <int L, typename TW>
 using ZMaxGeom = Geom<L, PlanSmall<L>::type::E, TW>;
@@ -609,15 +875,20 @@ int launch_inv_L(int mode, int lin, Prec ip, Prec wp, const void *in, void *out,
     if constexpr (L % 3 == 0 && (2 * L / 3) % 2 == 0) {
         constexpr int LIN = 2 * L / 3;
         if (lin != LIN) return 2;
-        if (mode != INV_U3DX && mode != INV_UOMEGA) return 2;
+        if (mode != INV_U3DX && mode != INV_UOMEGA && mode != INV_U3) return 2;
         if (ip == F64 && wp == F64) {
+            if (mode == INV_U3) return run_inv<L, LIN, INV_U3, double, double>(in, out, tw, g, n, ks, scale, st);
             return mode == INV_U3DX ? run_inv<L, LIN, INV_U3DX, double, double>(in, out, tw, g, n, ks, scale, st)
                                     : run_inv<L, LIN, INV_UOMEGA, double, double>(in, out, tw, g, n, ks, scale, st);
         }
         if (ip == F64 && wp == F32 && mode == INV_U3DX)
             return run_inv<L, LIN, INV_U3DX, double, float>(in, out, tw, g, n, ks, scale, st);
+        if (ip == F64 && wp == F32 && mode == INV_U3)
+            return run_inv<L, LIN, INV_U3, double, float>(in, out, tw, g, n, ks, scale, st);
         if (ip == F32 && wp == F32 && mode == INV_UOMEGA)
             return run_inv<L, LIN, INV_UOMEGA, float, float>(in, out, tw, g, n, ks, scale, st);
+        if (ip == F32 && wp == F32 && mode == INV_U3)
+            return run_inv<L, LIN, INV_U3, float, float>(in, out, tw, g, n, ks, scale, st);
     }
     return 2;
 }
@@ -654,6 +925,14 @@ int launch_fwd_L(int mode, int lout, int nf, Prec wp, const void *in, void *out,
             if (wp == F64) return run_fwd<L, LOUT, FWD_STAGE, 3, double>(in, out, tw, g, n, ks, scale, sa, st);
             return run_fwd<L, LOUT, FWD_STAGE, 3, float>(in, out, tw, g, n, ks, scale, sa, st);
         }
+        if (mode == FWD_PLAIN && nf == 6) {
+            if (wp == F64) return run_fwd<L, LOUT, FWD_PLAIN, 6, double>(in, out, tw, g, n, ks, scale, sa, st);
+            return run_fwd<L, LOUT, FWD_PLAIN, 6, float>(in, out, tw, g, n, ks, scale, sa, st);
+        }
+        if (mode == FWD_VJP && nf == 6 && sa) {
+            if (wp == F64) return run_fwd<L, LOUT, FWD_VJP, 6, double>(in, out, tw, g, n, ks, scale, sa, st);
+            return run_fwd<L, LOUT, FWD_VJP, 6, float>(in, out, tw, g, n, ks, scale, sa, st);
+        }
     }
     return 2;
 }
@@ -672,6 +951,26 @@ int launch_z_nl_L(Prec wp, const void *in, void *out, long long lines, int hp, l
             hipLaunchKernelGGL((k_z_nl<L, float>), blocks_for<void>(lines, G::T), dim3(G::NT), 0, st,
                                reinterpret_cast<const cpx<float> *>(in), reinterpret_cast<cpx<float> *>(out), lines,
                                hp, fs_in, fs_out, float(s_in), float(s_out), reinterpret_cast<const cpx<float> *>(tw));
+        }
+        return hipGetLastError() == hipSuccess ? 0 : 3;
+    }
+    return 2;
+}
+
+template <int L>
+int launch_z_vjp_L(Prec wp, const void *in, void *out, long long lines, int hp, long long fs_in, long long fs_out,
+                   double s_u, double s_o, const void *tw, hipStream_t st) {
+    if constexpr (L % 3 == 0 && (2 * L / 3) % 2 == 0) {
+        if (wp == F64) {
+            using G = ZNLGeom<L, double>;
+            hipLaunchKernelGGL((k_z_vjp<L, double>), blocks_for<void>(lines, G::T), dim3(G::NT), 0, st,
+                               reinterpret_cast<const cpx<double> *>(in), reinterpret_cast<cpx<double> *>(out), lines,
+                               hp, fs_in, fs_out, s_u, s_o, reinterpret_cast<const cpx<double> *>(tw));
+        } else {
+            using G = ZNLGeom<L, float>;
+            hipLaunchKernelGGL((k_z_vjp<L, float>), blocks_for<void>(lines, G::T), dim3(G::NT), 0, st,
+                               reinterpret_cast<const cpx<float> *>(in), reinterpret_cast<cpx<float> *>(out), lines,
+                               hp, fs_in, fs_out, float(s_u), float(s_o), reinterpret_cast<const cpx<float> *>(tw));
         }
         return hipGetLastError() == hipSuccess ? 0 : 3;
     }

@@ -51,8 +51,13 @@ int launch_inv(int mode, int L, int lin, Prec in_prec, Prec work_prec, const voi
 enum FwdMode {
     FWD_PLAIN = 0,  // nf fields, each truncated to lout positions
     FWD_STAGE = 1,  // 3 fields, truncate, Leray, then an ETD-RK4 stage epilogue
+    FWD_VJP = 2,    // 6 fields (GU, GW), truncate, t = GU + curl GW, then a reverse-stage epilogue
 };
-enum Stage { ST_RAW = 0, ST_1 = 1, ST_2 = 2, ST_3 = 3, ST_4 = 4 };
+enum Stage {
+    ST_RAW = 0, ST_1 = 1, ST_2 = 2, ST_3 = 3, ST_4 = 4,
+    // reverse sweep of the step (discrete adjoint), in the order they run
+    AS_RAW = 10, AS_4 = 14, AS_3 = 13, AS_2 = 12, AS_1 = 11,
+};
 struct StageArgs {
     int stage;
     void *out;        // ST_RAW: N hat; ST_1..3: next stage input; ST_4: u (in place)
@@ -63,6 +68,14 @@ struct StageArgs {
     const double *tab;  // ETD table, 7 arrays of (qmax+1) doubles
     int tab_stride;     // qmax+1
     const Scalars *sc;  // dt
+    // reverse sweep (AS_*): lam = cotangent of the step output, gu = cotangent
+    // accumulated on u, g1p = partial cotangent of N1, cot = next VJP input
+    // (projected and scaled), out = cotangent of the step input (AS_1)
+    const void *lam;
+    void *gu;
+    void *g1p;
+    void *cot;
+    double inv_scale;
 };
 int launch_fwd(int mode, int L, int lout, int nf, Prec work_prec, const void *in, void *out,
                const void *tw, const Strided &g, int n, double ks, double scale,
@@ -81,6 +94,17 @@ int launch_z_c2r(int L, Prec prec, const void *in, double *out, long long lines,
                  double scale, const void *tw, hipStream_t st);
 int launch_z_r2c(int L, Prec prec, const double *in, void *out, long long lines, int hp,
                  const void *tw, hipStream_t st);
+
+// VJP z pass: 9 padded fields (u, w, g) -> 6 truncated fields (GU, GW)
+int launch_z_vjp(int L, Prec work_prec, const void *in, void *out, long long lines, int hp, long long fs_in,
+                 long long fs_out, double s_u, double s_o, const void *tw, hipStream_t st);
+// cot = inv_scale * P(coef * lam) with coef = dt * b4 (use_b4) or 1, state layout
+int launch_adj_prep(const double *lam, double *cot, int n, int hp, long long fs, double ks, double inv_scale,
+                    const double *tab, int tab_stride, const Scalars *sc, int use_b4, hipStream_t st);
+// lam += coef * w * curl(curl(u)) (w = R2C weight), state layout: the gradient
+// of coef/2 * sum w |curl u|^2
+int launch_add_curlcurl(double *lam, const double *u, int n, int hp, long long fs, double ks, double coef,
+                        hipStream_t st);
 
 // helpers ---------------------------------------------------------------------
 int launch_repack(const void *src, void *dst, long long rows, int src_pitch, int dst_pitch,

@@ -15,6 +15,9 @@ template <int L>
 int launch_z_nl_L(Prec wp, const void *in, void *out, long long lines, int hp, long long fs_in,
                   long long fs_out, double s_in, double s_out, const void *tw, hipStream_t st);
 template <int L>
+int launch_z_vjp_L(Prec wp, const void *in, void *out, long long lines, int hp, long long fs_in,
+                   long long fs_out, double s_u, double s_o, const void *tw, hipStream_t st);
+template <int L>
 int launch_z_max_L(Prec wp, const void *in, long long lines, int hp, long long fs, double s_in,
                    const void *tw, Scalars *sc, hipStream_t st);
 template <int L>

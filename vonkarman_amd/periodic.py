@@ -253,6 +253,50 @@ class Periodic3D:
         check(lib().vk_solver_nonlinear(self._h, ptr(a), ptr(out)), "vk_solver_nonlinear")
         return out
 
+    # -- discrete adjoint (no reference counterpart) -------------------------
+    def nonlinear_vjp(self, u_hat: np.ndarray, lam: np.ndarray) -> np.ndarray:
+        """(d compute_nonlinear / d u)^T lam at u_hat (real inner product over
+        the stored entries; the result holds dJ/dRe + i dJ/dIm)."""
+        a = np.ascontiguousarray(u_hat, dtype=np.complex128)
+        l = np.ascontiguousarray(lam, dtype=np.complex128)
+        out = np.empty_like(a)
+        check(lib().vk_solver_nonlinear_vjp(self._h, ptr(a), ptr(l), ptr(out)), "vk_solver_nonlinear_vjp")
+        return out
+
+    def step_vjp(self, lam_out: np.ndarray) -> np.ndarray:
+        """(d step / d u)^T lam_out at the current state and dt."""
+        l = np.ascontiguousarray(lam_out, dtype=np.complex128)
+        out = np.empty_like(l)
+        check(lib().vk_solver_step_vjp(self._h, ptr(l), ptr(out)), "vk_solver_step_vjp")
+        return out
+
+    def adjoint(self, nsteps: int, objective: str | None = None, lam_T: np.ndarray | None = None,
+                max_snapshots: int = 40):
+        """Gradient of J with respect to the current state over `nsteps`
+        fixed-dt steps, with binomial checkpointing (at most `max_snapshots`
+        stored states).  objective: None, "final_enstrophy" or "dissipation"
+        (sum over steps n=1..T of dt 2 nu enstrophy(u_n)); lam_T seeds dJ/du_T.
+        Returns (J, grad, forward_steps); the solver is left at its state."""
+        obj = {None: 0, "final_enstrophy": 1, "dissipation": 2}[objective]
+        n = self.n
+        grad = np.empty((3, n, n, n // 2 + 1), np.complex128)
+        lt = None if lam_T is None else np.ascontiguousarray(lam_T, dtype=np.complex128)
+        J = ctypes.c_double(0.0)
+        fs = ctypes.c_int64(0)
+        check(lib().vk_solver_adjoint(self._h, nsteps, obj, None if lt is None else ptr(lt), max_snapshots,
+                                      ctypes.byref(J), ptr(grad), ctypes.byref(fs)), "vk_solver_adjoint")
+        return J.value, grad, fs.value
+
+    def adjoint_device(self, nsteps: int, objective: int, d_lam_T: int | None, max_snapshots: int,
+                       d_grad: int) -> tuple[float, int]:
+        J = ctypes.c_double(0.0)
+        fs = ctypes.c_int64(0)
+        check(lib().vk_solver_adjoint_device(self._h, nsteps, objective,
+                                             None if d_lam_T is None else ctypes.c_void_p(d_lam_T), max_snapshots,
+                                             ctypes.byref(J), ctypes.c_void_p(d_grad), ctypes.byref(fs)),
+              "vk_solver_adjoint_device")
+        return J.value, fs.value
+
     def snapshot(self) -> Snapshot:
         """Domain::snapshot (solver.rs:589-623)."""
         n = self.n
