@@ -234,10 +234,11 @@ VK_HD int bfly_tw(int j, int m) { return m * (j % NS) * (L / (NS * RAD)); }
 // Twiddle the inputs m = 1..RAD-1 of butterfly j: multiply by w^m (D < 0) or
 // conj(w)^m (D > 0), w = W_L^{(j % NS) L/(NS RAD)}.  Only w is read from the
 // table; its powers are formed by repeated multiplication (a few ulps).
-template <int L, int RAD, int NS, int D, typename R>
-VK_HD void twiddle_bfly(cpx<R> *v, int j, const cpx<R> *tw) {
+template <int L, int RAD, int NS, int D, typename R, typename TP>
+VK_HD void twiddle_bfly(cpx<R> *v, int j, TP tw) {
     if constexpr (NS > 1) {
-        const cpx<R> w1 = tw[bfly_tw<L, RAD, NS>(j, 1)];
+        const int i = bfly_tw<L, RAD, NS>(j, 1);
+        const cpx<R> w1 = {tw[i].re, tw[i].im};
         cpx<R> w = w1;
         for (int m = 1; m < RAD; ++m) {
             v[m] = D < 0 ? cmul(v[m], w) : cmulc(v[m], w);
@@ -383,9 +384,31 @@ struct Geom {
 };
 
 #if defined(__HIPCC__)
+// A pointer into global memory.  Loads through it are global_load (a generic
+// pointer gives flat_load, which also counts against the LDS wait counter, so
+// every LDS wait would wait for the HBM loads in flight too).
+template <typename T>
+using gptr = const __attribute__((address_space(1))) T *;
+
+// A global pointer the optimiser cannot see through (no instruction): address
+// arithmetic derived from it stays inside the loop iteration that uses it
+// instead of being hoisted, with its registers, out of a per-field loop.
+template <typename T>
+__device__ __forceinline__ gptr<T> opaque(const T *p) {
+    unsigned long long a = reinterpret_cast<unsigned long long>(p);
+    asm volatile("" : "+s"(a));
+    return (gptr<T>)a;
+}
+
+// load element i of a complex array through a global pointer
+template <typename R>
+__device__ __forceinline__ cpx<R> ldg(gptr<cpx<R>> p, long long i) {
+    return {p[i].re, p[i].im};
+}
+
 // one pass of the register-staged transform
 template <int L, int NS, int D, int T, int NT, int SP, int SC, int E, bool FIRST, typename R, int RAD, int... REST>
-__device__ __forceinline__ void fft_reg_run(cpx<R> (&v)[E], cpx<R> *lds, const cpx<R> *__restrict__ tw, int tid) {
+__device__ __forceinline__ void fft_reg_run(cpx<R> (&v)[E], cpx<R> *lds, gptr<cpx<R>> tw, int tid) {
     static_assert(E % RAD == 0, "E must be a multiple of every radix");
     constexpr int BPT = E / RAD;
     static_assert(T * (L / RAD) == BPT * NT, "butterflies must split evenly over the threads");
@@ -420,8 +443,8 @@ __device__ __forceinline__ void fft_reg_run(cpx<R> (&v)[E], cpx<R> *lds, const c
 }
 
 template <int L, int D, int T, int NT, int SP, int SC, int E, typename R, int... RS>
-__device__ __forceinline__ void fft_reg_list(RadixList<RS...>, cpx<R> (&v)[E], cpx<R> *lds,
-                                             const cpx<R> *__restrict__ tw, int tid) {
+__device__ __forceinline__ void fft_reg_list(RadixList<RS...>, cpx<R> (&v)[E], cpx<R> *lds, gptr<cpx<R>> tw,
+                                             int tid) {
     fft_reg_run<L, 1, D, T, NT, SP, SC, E, true, R, RS...>(v, lds, tw, tid);
 }
 
@@ -435,9 +458,7 @@ __device__ __forceinline__ void fft_reg(cpx<R> (&v)[E], cpx<R> *lds, const cpx<R
     // loaded where they are used instead of being hoisted (with their powers)
     // out of the callers' per-field loops, which would hold them all in
     // registers at once.
-    const cpx<R> *twp = tw;
-    asm volatile("" : "+s"(twp));
-    fft_reg_list<L, D, T, NT, SP, SC>(PL{}, v, lds, twp, tid);
+    fft_reg_list<L, D, T, NT, SP, SC>(PL{}, v, lds, opaque(tw), tid);
 }
 #endif
 

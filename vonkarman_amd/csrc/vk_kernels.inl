@@ -21,15 +21,6 @@ __device__ __forceinline__ cpx<TO> cvt(cpx<TI> a) {
     return {TO(a.re), TO(a.im)};
 }
 
-// Make a pointer opaque to the optimiser (no instruction): address arithmetic
-// derived from it stays inside the loop iteration that uses it instead of
-// being hoisted, with its registers, out of a per-field loop.
-template <typename P>
-__device__ __forceinline__ P *opaque(P *p) {
-    asm volatile("" : "+s"(p));
-    return p;
-}
-
 __device__ __forceinline__ long long col_addr(int col, int inner, long long os) {
     return (long long)(col / inner) * os + (col % inner);
 }
@@ -88,11 +79,11 @@ __global__ __launch_bounds__((InvGeom<L, MODE, TW>::NT)) void k_inv(const cpx<TI
     // fill a register slot with input field f (opaque pointer: the loads stay
     // after the previous transform's barrier)
     auto fill = [&](cpx<TW>(&S)[E], int f) {
-        const cpx<TI> *src = opaque(in) + f * g.in_fs + cin;
+        const gptr<cpx<TI>> src = opaque(in) + f * g.in_fs + cin;
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const int i = uidx(e);
-            S[e] = (colok && i >= 0) ? cvt<TW>(src[(long long)i * g.in_ls]) : zero;
+            S[e] = (colok && i >= 0) ? cvt<TW>(ldg(src, (long long)i * g.in_ls)) : zero;
         }
     };
     auto kline = [&](int e) -> TW {
@@ -328,11 +319,11 @@ __global__ __launch_bounds__((FwdGeom<L, MODE, TW>::NT)) void k_fwd(const cpx<TW
     const long long cin = colok ? col_addr(col, g.in_inner, g.in_os) : 0;
     const long long cout = colok ? col_addr(col, g.out_inner, g.out_os) : 0;
     auto load_field = [&](int f) {
-        const cpx<TW> *src = opaque(in) + f * g.in_fs + cin;
+        const gptr<cpx<TW>> src = opaque(in) + f * g.in_fs + cin;
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const int p = ci_pos_in<L, PL::first, T, NT>(tid, e);
-            v[e] = colok ? src[(long long)p * g.in_ls] : zero;
+            v[e] = colok ? ldg(src, (long long)p * g.in_ls) : zero;
         }
     };
     // wavenumbers of kept element (b, mk)
@@ -471,22 +462,20 @@ __global__ __launch_bounds__((FwdGeom<L, MODE, TW>::NT)) void k_fwd(const cpx<TW
 // Hermitian-extended pair spectrum Z at FFT position p from the half
 // spectra fa, fb (HIN stored entries; fb may be null)
 template <int L, int HIN, typename TW>
-__device__ __forceinline__ cpx<TW> z_pair_at(const cpx<TW> *__restrict__ fa, const cpx<TW> *__restrict__ fb,
-                                            long long base, int p) {
+__device__ __forceinline__ cpx<TW> z_pair_at(gptr<cpx<TW>> fa, gptr<cpx<TW>> fb, long long base, int p) {
     if (p == 0 || 2 * p == L) {
         if (p >= HIN) return {TW(0), TW(0)};
-        const cpx<TW> X = fa[base + p];
-        return {X.re, fb ? fb[base + p].re : TW(0)};
+        return {fa[base + p].re, fb ? fb[base + p].re : TW(0)};
     }
     if (p < HIN) {
-        const cpx<TW> X = fa[base + p];
-        const cpx<TW> Y = fb ? fb[base + p] : cpx<TW>{TW(0), TW(0)};
+        const cpx<TW> X = ldg(fa, base + p);
+        const cpx<TW> Y = fb ? ldg(fb, base + p) : cpx<TW>{TW(0), TW(0)};
         return {X.re - Y.im, X.im + Y.re};
     }
     if (p > L - HIN) {
         const int k = L - p;
-        const cpx<TW> X = fa[base + k];
-        const cpx<TW> Y = fb ? fb[base + k] : cpx<TW>{TW(0), TW(0)};
+        const cpx<TW> X = ldg(fa, base + k);
+        const cpx<TW> Y = fb ? ldg(fb, base + k) : cpx<TW>{TW(0), TW(0)};
         return {X.re + Y.im, Y.re - X.im};
     }
     return {TW(0), TW(0)};
@@ -523,11 +512,11 @@ __global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) void k_z_nl(const cpx<TW> *__
     // fields: 0 ux, 1 uy, 2 uz, 3 wx, 4 wy, 5 wz; pairs (0,1) (2,3) (4,5)
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-        const cpx<TW> *fa = opaque(in) + (2 * q) * fs_in, *fb = in + (2 * q + 1) * fs_in;
+        const gptr<cpx<TW>> fa = opaque(in) + (2 * q) * fs_in, fb = opaque(in) + (2 * q + 1) * fs_in;
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const int p = ci_pos_in<L, PL::first, T, NT>(tid, e);
-            v[e] = ok ? z_pair_at<L, HIN>(fa, fb, base, p) : cpx<TW>{TW(0), TW(0)};
+            v[e] = ok ? z_pair_at<L, HIN, TW>(fa, fb, base, p) : cpx<TW>{TW(0), TW(0)};
         }
         fft_reg<L, +1, T, NT, T, 1, PL>(v, lds, tw, tid);
 #pragma unroll
@@ -582,9 +571,9 @@ __global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) void k_z_nl(const cpx<TW> *__
 // Pair spectrum with per-half DC factors (the z transpose of an R2C counts
 // its k=0 term once where the C2R extension would count it twice).
 template <int L, int HIN, typename TW>
-__device__ __forceinline__ cpx<TW> z_pair_dc(const cpx<TW> *__restrict__ fa, const cpx<TW> *__restrict__ fb,
-                                            long long base, int p, TW dca, TW dcb) {
-    const cpx<TW> z = z_pair_at<L, HIN>(fa, fb, base, p);
+__device__ __forceinline__ cpx<TW> z_pair_dc(gptr<cpx<TW>> fa, gptr<cpx<TW>> fb, long long base, int p, TW dca,
+                                            TW dcb) {
+    const cpx<TW> z = z_pair_at<L, HIN, TW>(fa, fb, base, p);
     return p == 0 ? cpx<TW>{z.re * dca, z.im * dcb} : z;
 }
 
@@ -636,11 +625,11 @@ __global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) void k_z_vjp(const cpx<TW> *_
     TW g0[E], g1[E], r0[E], r1[E], r2[E], r3[E];
 
     auto inverse = [&](int fa, int fb, TW dca, TW dcb) {
-        const cpx<TW> *pa = opaque(in) + fa * fs_in, *pb = fb >= 0 ? in + fb * fs_in : nullptr;
+        const gptr<cpx<TW>> pa = opaque(in) + fa * fs_in, pb = fb >= 0 ? opaque(in) + fb * fs_in : nullptr;
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const int p = ci_pos_in<L, PL::first, T, NT>(tid, e);
-            v[e] = ok ? z_pair_dc<L, HIN>(pa, pb, base, p, dca, dcb) : cpx<TW>{TW(0), TW(0)};
+            v[e] = ok ? z_pair_dc<L, HIN, TW>(pa, pb, base, p, dca, dcb) : cpx<TW>{TW(0), TW(0)};
         }
         fft_reg<L, +1, T, NT, T, 1, PL>(v, lds, tw, tid);
     };
@@ -726,11 +715,11 @@ __global__ __launch_bounds__((ZMaxGeom<L, TW>::NT)) void k_z_max(const cpx<TW> *
     TW r[3][E];
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
-        const cpx<TW> *fa = opaque(in) + (2 * q) * fs, *fb = q == 0 ? in + fs : nullptr;
+        const gptr<cpx<TW>> fa = opaque(in) + (2 * q) * fs, fb = q == 0 ? opaque(in) + fs : nullptr;
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const int p = ci_pos_in<L, PL::first, T, NT>(tid, e);
-            v[e] = ok ? z_pair_at<L, HIN>(fa, fb, base, p) : cpx<TW>{TW(0), TW(0)};
+            v[e] = ok ? z_pair_at<L, HIN, TW>(fa, fb, base, p) : cpx<TW>{TW(0), TW(0)};
         }
         fft_reg<L, +1, T, NT, T, 1, PL>(v, lds, tw, tid);
 #pragma unroll
@@ -781,9 +770,9 @@ __global__ __launch_bounds__((ZSGeom<L, TW>::NT)) void k_z_c2r(const cpx<TW> *__
         const int p = ci_pos_in<L, PL::first, T, NT>(tid, e);
         v[e] = {TW(0), TW(0)};
         if (la < lines) {
-            const cpx<TW> za = z_pair_at<L, HIN>(in, (const cpx<TW> *)nullptr, la * hp, p);
-            const cpx<TW> zb =
-                lb < lines ? z_pair_at<L, HIN>(in, (const cpx<TW> *)nullptr, lb * hp, p) : cpx<TW>{TW(0), TW(0)};
+            const gptr<cpx<TW>> src = opaque(in), none = nullptr;
+            const cpx<TW> za = z_pair_at<L, HIN, TW>(src, none, la * hp, p);
+            const cpx<TW> zb = lb < lines ? z_pair_at<L, HIN, TW>(src, none, lb * hp, p) : cpx<TW>{TW(0), TW(0)};
             v[e] = {za.re - zb.im, za.im + zb.re};  // X_a + i X_b
         }
     }
