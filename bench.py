@@ -134,6 +134,281 @@ def run_gpu(args):
     return n, out, timing, fft, {"energy_rel_err": abs(emx - e64) / e64, "field_rel_l2": field_rel}
 
 
+def adjoint_cost(n: int, nsteps: int = 8) -> dict:
+    """Wall time of the discrete adjoint against the forward steps it
+    differentiates: FP64, fixed dt (the CFL dt of the IC), J = time-integrated
+    dissipation.  With nsteps snapshots every state is stored (one forward
+    sweep); with 2 the binomial schedule recomputes."""
+    import ctypes
+
+    from vonkarman_amd import GridSpec, IcType, Periodic3D, _lib
+    from vonkarman_amd import ic as vic
+
+    u0 = vic.von_karman_isotropic(n, k0=4.0, kd=64.0, energy=0.5, seed=42)
+    s = Periodic3D(GridSpec.cubic(n), 1e-3, IcType.Spectral(u0))
+    s.set_fixed_dt(s.dt())
+    s.steps(1)  # warm (kernels loaded, ETD table built)
+    s.set_u_hat(u0)
+    s.set_clock(0.0, 0)
+    t0 = time.perf_counter()
+    s.steps(nsteps)
+    t_fwd = time.perf_counter() - t0
+    s.set_u_hat(u0)
+    s.set_clock(0.0, 0)
+    res = {"n": n, "steps": nsteps, "dt": s.dt(), "objective": "dissipation", "forward_s": t_fwd}
+    L = _lib.lib()
+    for snaps in (1 + nsteps, 2):
+        J = ctypes.c_double(0.0)
+        fs = ctypes.c_int64(0)
+        _lib.check(L.vk_solver_adjoint(s._h, nsteps, _lib.VK_OBJ_DISSIPATION, None, snaps, ctypes.byref(J), None,
+                                       ctypes.byref(fs)), "vk_solver_adjoint")  # warm (buffers allocated)
+        t0 = time.perf_counter()
+        _lib.check(L.vk_solver_adjoint(s._h, nsteps, _lib.VK_OBJ_DISSIPATION, None, snaps, ctypes.byref(J), None,
+                                       ctypes.byref(fs)), "vk_solver_adjoint")
+        t = time.perf_counter() - t0
+        key = "all_states" if snaps > nsteps else "two_snapshots"
+        res[key] = {"snapshots": min(snaps, nsteps), "adjoint_s": t, "forward_steps": fs.value,
+                    "ratio_vs_forward": t / t_fwd}
+    res["device_bytes"] = s.device_bytes()
+    print(f"[bench] adjoint: {nsteps} steps, ratio {res['all_states']['ratio_vs_forward']:.2f} "
+          f"(two snapshots {res['two_snapshots']['ratio_vs_forward']:.2f})", file=sys.stderr, flush=True)
+    del s
+    return res
+
+
+def _rel(a, b) -> float:
+    import numpy as np
+
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def workload_mixed_error(sizes=(256, 512), nsteps: int = 1000) -> dict:
+    """Mixed precision against FP64 on the GPU: the von Karman field at 256^3
+    (nu=1e-3) and 512^3 (nu=4e-4), fixed dt=5e-4, nsteps steps; E(t) and Z(t)
+    every 10 steps, E(k) and the final field at the end."""
+    import numpy as np
+
+    from vonkarman_amd import GridSpec, IcType, Periodic3D
+    from vonkarman_amd import ic as vic
+
+    out = {"workload": "mixed vs FP64, von Karman isotropic (k0=4, kd=64, E=0.5, seed 42), fixed dt=5e-4",
+           "steps": nsteps}
+    for n in sizes:
+        nu = 1e-3 if n <= 256 else 4e-4
+        u0 = vic.von_karman_isotropic(n, k0=4.0, kd=64.0, energy=0.5, seed=42)
+        sol = {}
+        for prec in ("f64", "mixed"):
+            sol[prec] = Periodic3D(GridSpec.cubic(n), nu, IcType.Spectral(u0), precision=prec, init_dt=False)
+            sol[prec].set_fixed_dt(5e-4)
+        del u0
+        E = {p: [] for p in sol}
+        Z = {p: [] for p in sol}
+        secs = {p: 0.0 for p in sol}
+        done = 0
+        while done < nsteps:
+            k = min(10, nsteps - done)
+            for p, s in sol.items():
+                t0 = time.perf_counter()
+                s.steps(k)
+                secs[p] += time.perf_counter() - t0
+                d = s.diagnostics_struct()
+                E[p].append(d.energy)
+                Z[p].append(d.enstrophy)
+            done += k
+            if done % 100 == 0:
+                print(f"[bench] mixed-error n={n}: {done}/{nsteps} steps, E64={E['f64'][-1]:.9f} "
+                      f"Emx={E['mixed'][-1]:.9f}", file=sys.stderr, flush=True)
+        e64, emx = np.array(E["f64"]), np.array(E["mixed"])
+        z64, zmx = np.array(Z["f64"]), np.array(Z["mixed"])
+        k64, kmx = sol["f64"].energy_spectrum_array(), sol["mixed"].energy_spectrum_array()
+        sig = k64 > 1e-12 * k64.max()
+        out[str(n)] = {
+            "nu": nu,
+            "steps_per_s_f64": nsteps / secs["f64"], "steps_per_s_mixed": nsteps / secs["mixed"],
+            "energy_max_rel_err": float(np.max(np.abs(emx - e64) / e64)),
+            "enstrophy_max_rel_err": float(np.max(np.abs(zmx - z64) / z64)),
+            "spectrum_rel_l2": _rel(kmx, k64),
+            "spectrum_max_shell_rel_err": float(np.max(np.abs(kmx[sig] - k64[sig]) / k64[sig])),
+            "final_field_rel_l2": _rel(sol["mixed"].u_hat(), sol["f64"].u_hat()),
+            "energy_final_f64": float(e64[-1]), "samples": len(e64),
+        }
+        print(f"[bench] mixed-error n={n}: {json.dumps(out[str(n)])}", file=sys.stderr, flush=True)
+        del sol
+    return out
+
+
+def _solenoidal_direction(n: int, seed: int):
+    """A smooth seeded divergence-free direction (Leray-projected random
+    modes with |k| <= 8, Hermitian by construction: the R2C of a real field)."""
+    import numpy as np
+
+    from vonkarman_amd import HipFftBackend
+
+    rng = np.random.default_rng(seed)
+    x = np.arange(n) * 2 * math.pi / n
+    fft = HipFftBackend(n, n, n)
+    out = np.empty((3, n, n, n // 2 + 1), np.complex128)
+    for c in range(3):
+        f = np.zeros((n, n, n))
+        for _ in range(6):
+            k = rng.integers(-8, 9, size=3)
+            f += rng.standard_normal() * np.cos(k[0] * x[:, None, None] + k[1] * x[None, :, None]
+                                                 + k[2] * x[None, None, :] + rng.uniform(0, 2 * math.pi))
+        fft.r2c_3d(f, out[c])
+    kk = np.fft.fftfreq(n, 1.0 / n)
+    kx, ky, kz = kk[:, None, None], kk[None, :, None], np.arange(n // 2 + 1)[None, None, :]
+    kx = np.where(np.arange(n)[:, None, None] == n // 2, -n // 2, kx)
+    ky = np.where(np.arange(n)[None, :, None] == n // 2, -n // 2, ky)
+    k2 = kx ** 2 + ky ** 2 + kz ** 2
+    dot = kx * out[0] + ky * out[1] + kz * out[2]
+    k2s = np.where(k2 == 0, 1.0, k2)
+    for c, kc in enumerate((kx, ky, kz)):
+        out[c] = out[c] - np.where(k2 == 0, 0.0, kc * dot / k2s)
+    return out
+
+
+def workload_adjoint_tubes(n: int = 128, nsteps: int = 200, ndir: int = 8) -> dict:
+    """Anti-parallel vortex tubes, FP64, nu=1e-3, dt=2e-3, J = final
+    enstrophy: gradient by the discrete adjoint (10 snapshots), central
+    differences (eps = 1e-6 relative) along seeded divergence-free directions,
+    and a 3-step comparison with the oracle's CPU adjoint."""
+    import numpy as np
+
+    from vonkarman_amd import GridSpec, IcType, Periodic3D
+
+    nu, dt = 1e-3, 2e-3
+    s = Periodic3D(GridSpec.cubic(n), nu, IcType.AntiParallelTubes(), init_dt=False)
+    s.set_fixed_dt(dt)
+    u0 = s.u_hat()
+    s.steps(1)
+    s.set_u_hat(u0)
+    s.set_clock(0.0, 0)
+    t0 = time.perf_counter()
+    s.steps(nsteps)
+    t_fwd = time.perf_counter() - t0
+    s.set_u_hat(u0)
+    s.set_clock(0.0, 0)
+    t0 = time.perf_counter()
+    J, g, fsteps = s.adjoint(nsteps, "final_enstrophy", max_snapshots=10)
+    t_adj = time.perf_counter() - t0
+    print(f"[bench] tubes: J={J:.10e} forward {t_fwd:.2f} s adjoint {t_adj:.2f} s", file=sys.stderr, flush=True)
+
+    def J_of(u):
+        s.set_u_hat(u)
+        s.set_clock(0.0, 0)
+        s.steps(nsteps)
+        return s.enstrophy()
+
+    fd = []
+    for i in range(ndir):
+        d = _solenoidal_direction(n, 1000 + i)
+        eps = 1e-6 * np.linalg.norm(u0) / np.linalg.norm(d)
+        jp, jm = J_of(u0 + eps * d), J_of(u0 - eps * d)
+        fdv = (jp - jm) / (2 * eps)
+        adv = float(np.vdot(g, d).real)
+        fd.append({"direction": i, "fd": fdv, "adjoint": adv, "rel_err": abs(fdv - adv) / abs(fdv)})
+        print(f"[bench] tubes fd {i}: {fd[-1]}", file=sys.stderr, flush=True)
+    s.set_u_hat(u0)
+    s.set_clock(0.0, 0)
+    # oracle CPU adjoint over a short horizon (3 steps), same objective
+    from oracle import oracle as O
+
+    O.lib().or_set_threads(min(16, os.cpu_count() or 1))
+    T3 = 3
+    J3, g3, _ = s.adjoint(T3, "final_enstrophy", max_snapshots=T3)
+    r = O.Solver(n, nu)
+    r.set_state(u0)
+    r.set_dt(dt, True)
+    states = [u0]
+    for _ in range(T3):
+        r.step(1)
+        states.append(r.state())
+    J_cpu = r.enstrophy()  # at u_T3 (the VJP loop below moves the oracle's state)
+    w = np.full(n // 2 + 1, 2.0)
+    w[0] = w[n // 2] = 1.0
+    lam = 2.0 * w * O.curl(O.curl(states[T3])) / float(n) ** 6
+    for k in range(T3 - 1, -1, -1):
+        r.set_state(states[k])
+        lam = r.step_vjp(lam)
+    cpu = {"steps": T3, "J_rel_err": abs(J3 - J_cpu) / J_cpu, "grad_rel_l2": _rel(g3, lam)}
+    print(f"[bench] tubes vs oracle adjoint: {cpu}", file=sys.stderr, flush=True)
+    return {"workload": f"adjoint, anti-parallel tubes {n}^3 ({3 * n // 2}^3 padded), FP64, nu={nu}, dt={dt}, "
+                        f"{nsteps} steps, J = final enstrophy", "J": J, "forward_s": t_fwd, "adjoint_s": t_adj,
+            "ratio_vs_forward": t_adj / t_fwd, "forward_steps_in_adjoint": fsteps, "snapshots": 10,
+            "fd": fd, "fd_max_rel_err": max(x["rel_err"] for x in fd), "cpu_adjoint": cpu,
+            "device_bytes": s.device_bytes()}
+
+
+def workload_adjoint_iso_mixed(n: int = 256, nsteps: int = 2000, nsnap: int = 40, ndir: int = 2) -> dict:
+    """Mixed-precision adjoint at 256^3, von Karman field, dt=5e-4, nu=1e-3,
+    J = time-integrated dissipation, at most 40 FP64 snapshots: gradient error
+    against the FP64 adjoint, central differences of the FP64 forward, peak
+    device memory and the adjoint/forward wall-time ratio."""
+    import numpy as np
+
+    from vonkarman_amd import GridSpec, IcType, Periodic3D
+    from vonkarman_amd import ic as vic
+
+    nu, dt = 1e-3, 5e-4
+    u0 = vic.von_karman_isotropic(n, k0=4.0, kd=64.0, energy=0.5, seed=42)
+    res = {"workload": f"adjoint, von Karman {n}^3 ({3 * n // 2}^3 padded), mixed, nu={nu}, dt={dt}, {nsteps} steps, "
+                       f"J = sum_n dt 2 nu Omega(u_n), <= {nsnap} snapshots"}
+    grads = {}
+    for prec in ("mixed", "f64"):
+        s = Periodic3D(GridSpec.cubic(n), nu, IcType.Spectral(u0), precision=prec, init_dt=False)
+        s.set_fixed_dt(dt)
+        s.steps(1)
+        s.set_u_hat(u0)
+        s.set_clock(0.0, 0)
+        t0 = time.perf_counter()
+        done = 0
+        while done < nsteps:
+            s.steps(min(250, nsteps - done))
+            done += min(250, nsteps - done)
+        t_fwd = time.perf_counter() - t0
+        s.set_u_hat(u0)
+        s.set_clock(0.0, 0)
+        print(f"[bench] iso {prec}: forward {nsteps} steps {t_fwd:.1f} s; adjoint starts", file=sys.stderr,
+              flush=True)
+        t0 = time.perf_counter()
+        J, g, fsteps = s.adjoint(nsteps, "dissipation", max_snapshots=nsnap)
+        t_adj = time.perf_counter() - t0
+        grads[prec] = g
+        res[prec] = {"J": J, "forward_s": t_fwd, "adjoint_s": t_adj, "ratio_vs_forward": t_adj / t_fwd,
+                     "forward_steps_in_adjoint": fsteps, "device_bytes": s.device_bytes()}
+        print(f"[bench] iso {prec}: {res[prec]}", file=sys.stderr, flush=True)
+        if prec == "f64":
+            fd = []
+            for i in range(ndir):
+                d = _solenoidal_direction(n, 2000 + i)
+                eps = 1e-6 * np.linalg.norm(u0) / np.linalg.norm(d)
+                jp = _dissipation_only(s, u0 + eps * d, nsteps, dt, nu)
+                jm = _dissipation_only(s, u0 - eps * d, nsteps, dt, nu)
+                fdv = (jp - jm) / (2 * eps)
+                fd.append({"direction": i, "fd": fdv, "adjoint_f64": float(np.vdot(g, d).real),
+                           "adjoint_mixed": float(np.vdot(grads["mixed"], d).real)})
+                fd[-1]["rel_err_f64"] = abs(fd[-1]["adjoint_f64"] - fdv) / abs(fdv)
+                fd[-1]["rel_err_mixed"] = abs(fd[-1]["adjoint_mixed"] - fdv) / abs(fdv)
+                print(f"[bench] iso fd {i}: {fd[-1]}", file=sys.stderr, flush=True)
+            res["fd"] = fd
+        del s
+    res["grad_rel_l2_mixed_vs_f64"] = _rel(grads["mixed"], grads["f64"])
+    return res
+
+
+def _dissipation_only(s, u, nsteps, dt, nu) -> float:
+    """sum_{n=1..T} dt 2 nu Omega(u_n) by the forward solver alone."""
+    s.set_u_hat(u)
+    s.set_clock(0.0, 0)
+    tot = 0.0
+    for i in range(nsteps):
+        s.steps(1)
+        tot += dt * 2.0 * nu * s.enstrophy()
+        if (i + 1) % 250 == 0:
+            print(f"[bench] iso fd: {i + 1}/{nsteps} steps", file=sys.stderr, flush=True)
+    return tot
+
+
 KERNEL_OF_PASS = {  # kernel-name prefixes of the FP64 passes at the padded length M
     "x_inverse": "k_inv<{M}, {N}, 2, double, double>",
     "y_inverse": "k_inv<{M}, {N}, 4, double, double>",
@@ -232,6 +507,10 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-traffic", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--workload", default="step", choices=["step", "mixed-error", "adjoint-tubes", "adjoint-iso-mixed"],
+                    help="step: the throughput line (default); the others run the accuracy/adjoint configurations "
+                         "and print their own JSON line")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pmc_child:
@@ -241,6 +520,11 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU slab decomposition is a design note only)", file=sys.stderr)
         raise SystemExit(2)
+    if args.workload != "step":
+        fn = {"mixed-error": workload_mixed_error, "adjoint-tubes": workload_adjoint_tubes,
+              "adjoint-iso-mixed": workload_adjoint_iso_mixed}[args.workload]
+        print(json.dumps(fn()), flush=True)
+        return
 
     n, out, timing, fft, acc = run_gpu(args)
     f64 = out["f64"]
@@ -251,6 +535,7 @@ def main():
     dom = max(avg, key=lambda k: avg[k])
     achieved = pb[dom] / (avg[dom] * 1e-3) / 1e9
     traffic = None if args.no_traffic else measure_traffic(n, dom)
+    adj = adjoint_cost(n) if not args.no_adjoint else None
     cpu = None if args.no_cpu else cpu_baseline(n)
     sb = step_bytes(n, 8)
     line = {
@@ -282,6 +567,7 @@ def main():
                   "energy_rel_err_vs_f64": acc["energy_rel_err"], "field_rel_l2_vs_f64": acc["field_rel_l2"],
                   "steps_compared": args.warmup + args.steps + timing["f64"]["steps"]},
         "fft_c2r": fft,
+        "adjoint": adj,
     }
     print(json.dumps(line), flush=True)
 

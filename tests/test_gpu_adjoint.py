@@ -84,12 +84,29 @@ def test_step_vjp_matches_oracle(hip_device, n):
     assert rel(sol.u_hat(), u) == 0.0  # the VJP leaves the state
 
 
-def _gpu_solver(n, nu, dt, u):
+def _gpu_solver(n, nu, dt, u, precision="f64"):
     from vonkarman_amd import GridSpec, IcType, Periodic3D
 
-    s = Periodic3D(GridSpec.cubic(n), nu, IcType.Spectral(u), init_dt=False)
+    s = Periodic3D(GridSpec.cubic(n), nu, IcType.Spectral(u), init_dt=False, precision=precision)
     s.set_fixed_dt(dt)
     return s
+
+
+@pytest.mark.parametrize("n", [16, 32])
+def test_mixed_precision_vjps_close_to_f64(hip_device, n):
+    """FP32 transforms in the VJPs: the step VJP and a 4-step trajectory
+    gradient stay within FP32 rounding (1e-5 relative) of FP64."""
+    print(f"case mixed vjp n={n}", flush=True)
+    nu, dt = 0.01, 4e-3
+    u = hermitian_field(n, 21)
+    lam = crandn(np.random.default_rng(2), u.shape)
+    s64, smx = _gpu_solver(n, nu, dt, u), _gpu_solver(n, nu, dt, u, "mixed")
+    assert rel(smx.step_vjp(lam), s64.step_vjp(lam)) < 1e-5
+    assert rel(smx.nonlinear_vjp(u, lam), s64.nonlinear_vjp(u, lam)) < 1e-5
+    J64, g64, _ = s64.adjoint(4, "dissipation", max_snapshots=2)
+    Jmx, gmx, _ = smx.adjoint(4, "dissipation", max_snapshots=2)
+    assert abs(Jmx - J64) < 1e-5 * J64
+    assert rel(gmx, g64) < 1e-5
 
 
 def test_checkpoint_schedules_give_identical_gradients(hip_device):

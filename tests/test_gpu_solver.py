@@ -117,3 +117,30 @@ def test_adaptive_cfl_steps_match_oracle(hip_device):
     r.step(5)
     assert abs(s.dt() - r.dt) <= 1e-13 * r.dt
     assert _rel(s.u_hat(), r.state()) < 1e-12
+
+
+@pytest.mark.parametrize("n", [32, 64])
+def test_mixed_precision_tracks_fp64(hip_device, n):
+    """Mixed mode (FP32 transforms, products, projection and state; FP64
+    coefficients and reductions): Taylor-Green, 100 fixed steps of dt=1e-3 at
+    nu=6.25e-4.  Energy and enstrophy stay within 1e-6 of FP64, the field within
+    1e-5, and the nonlinear term within FP32 rounding."""
+    from vonkarman_amd import GridSpec, IcType, Periodic3D
+
+    print(f"case mixed vs f64 n={n}", flush=True)
+    u0 = _tg_state(n)
+    sol = {}
+    for prec in ("f64", "mixed"):
+        s = Periodic3D(GridSpec.cubic(n), 6.25e-4, IcType.Spectral(u0), precision=prec, init_dt=False)
+        s.set_fixed_dt(1e-3)
+        sol[prec] = s
+    assert _rel(sol["mixed"].nonlinear(u0), sol["f64"].nonlinear(u0)) < 2e-6
+    for _ in range(10):
+        for s in sol.values():
+            s.steps(10)
+        e64, emx = sol["f64"].energy(), sol["mixed"].energy()
+        z64, zmx = sol["f64"].enstrophy(), sol["mixed"].enstrophy()
+        assert abs(emx - e64) < 1e-6 * e64
+        assert abs(zmx - z64) < 1e-6 * z64
+    assert _rel(sol["mixed"].u_hat(), sol["f64"].u_hat()) < 1e-5
+    assert sol["mixed"].step_count() == 100

@@ -10,7 +10,8 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "libvonkarman_hip.so")
+# VONKARMAN_HIP_LIB selects another build of the library (e.g. one with other tile settings)
+LIB_PATH = os.environ.get("VONKARMAN_HIP_LIB") or os.path.join(_HERE, "lib", "libvonkarman_hip.so")
 CSRC = os.path.join(_HERE, "csrc")
 
 VK_OK = 0

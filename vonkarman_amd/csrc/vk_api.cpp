@@ -319,6 +319,9 @@ struct vk_solver {
     int split_T = 0, split_S = 0;
     Scalars *dsc() { return static_cast<Scalars *>(sc.p); }
     size_t wbytes() const { return wp == F64 ? 16 : 8; }
+    // state-layout arrays (u, x, n1, sacc, stage recomputes, snapshots) are in
+    // the work precision; the adjoint's cotangent arrays are FP64 in both modes
+    size_t state_bytes() const { return (size_t)S * 3 * wbytes(); }
 };
 
 namespace {
@@ -336,7 +339,7 @@ int nl_eval(vk_solver *s, const void *src, StageArgs &sa, vk_timing *t, hipEvent
     if (int r = mark(0)) return r;
     // P1: x inverse with padding: u[3][N][N][Hp] -> A[5][N][M][Hp] (y-major)
     Strided g1 = make_strided((int)NH, Hp, H, (int)NH, 0, NH, s->S, Hp, MH, Hp, NMH);
-    VK_TRY(launch_inv(INV_U3DX, M, N, F64, s->wp, src, s->A.p, s->tw_m.p, g1, N, s->ks, 1.0, s->st));
+    VK_TRY(launch_inv(INV_U3DX, M, N, s->wp, s->wp, src, s->A.p, s->tw_m.p, g1, N, s->ks, 1.0, s->st));
     if (int r = mark(1)) return r;
     // P2: y inverse with padding, curl completed: A -> B[6][M][M][Hp]
     Strided g2 = make_strided((int)MH, Hp, H, (int)MH, 0, MH, NMH, Hp, MH, Hp, MMH);
@@ -364,7 +367,7 @@ int max_pass(vk_solver *s, bool omega) {
     const long long NH = (long long)N * Hp, NNH = (long long)N * N * Hp;
     VK_TRY(launch_fill_u64(&s->dsc()->umax2_bits, 0ull, 1, s->st));
     Strided g1 = make_strided((int)NH, Hp, H, (int)NH, 0, NH, s->S, Hp, NH, Hp, NNH);
-    VK_TRY(launch_inv(omega ? INV_U3DX : INV_U3, N, N, F64, s->wp, s->u.p, s->A.p, s->tw_n.p, g1, N, s->ks, 1.0,
+    VK_TRY(launch_inv(omega ? INV_U3DX : INV_U3, N, N, s->wp, s->wp, s->u.p, s->A.p, s->tw_n.p, g1, N, s->ks, 1.0,
                       s->st));
     Strided g2 = make_strided((int)NH, Hp, H, (int)NH, 0, NH, NNH, Hp, NH, Hp, NNH);
     VK_TRY(launch_inv(omega ? INV_OMEGA : INV_U3, N, N, s->wp, s->wp, s->A.p, s->B.p, s->tw_n.p, g2, N, s->ks, 1.0,
@@ -470,7 +473,7 @@ int vk_solver_create(int n, double l, double nu, int precision, vk_solver **out)
     s->nshells = (int)std::ceil(kmax) + 1;  // spectrum.rs:26-30
     s->nblocks_diag = 1024;
     int r = 0;
-    const size_t state_bytes = (size_t)s->S * 3 * 16;
+    const size_t state_bytes = s->state_bytes();
     if ((r = dev_alloc(s->u, state_bytes, s->total)) || (r = dev_alloc(s->x, state_bytes, s->total)) ||
         (r = dev_alloc(s->n1, state_bytes, s->total)) || (r = dev_alloc(s->sacc, state_bytes, s->total)) ||
         (r = dev_alloc(s->A, a_elems * s->wbytes(), s->total)) ||
@@ -526,15 +529,19 @@ static int set_state_impl(vk_solver *s, const double *uhat, bool device) {
     if (!uhat) return fail(VK_ERR_ARG, "null state pointer");
     const size_t bytes = (size_t)3 * s->n * s->n * s->h * 16;
     VK_HIP(hipMemcpyAsync(s->staging.p, uhat, bytes, device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s->st));
-    VK_TRY(launch_repack(s->staging.p, s->u.p, (long long)3 * s->n * s->n, s->h, s->hp, s->h, 1, s->st));
+    VK_TRY(launch_repack_p(s->staging.p, F64, s->u.p, s->wp, (long long)3 * s->n * s->n, s->h, s->hp, s->h, 1, s->st));
     VK_HIP(hipStreamSynchronize(s->st));
     return VK_OK;
 }
-static int get_state_impl(vk_solver *s, double *uhat, bool device, const void *src = nullptr) {
+// copy a state-layout array (src, precision sprec; the state u by default) to
+// caller memory as unpadded complex double
+static int get_state_impl(vk_solver *s, double *uhat, bool device, const void *src = nullptr, int sprec = -1) {
     if (int r = check(s)) return r;
     if (!uhat) return fail(VK_ERR_ARG, "null state pointer");
     const size_t bytes = (size_t)3 * s->n * s->n * s->h * 16;
-    VK_TRY(launch_repack(src ? src : s->u.p, s->staging.p, (long long)3 * s->n * s->n, s->hp, s->h, s->h, 1, s->st));
+    const Prec sp = sprec < 0 ? s->wp : (Prec)sprec;
+    VK_TRY(launch_repack_p(src ? src : s->u.p, sp, s->staging.p, F64, (long long)3 * s->n * s->n, s->hp, s->h, s->h, 1,
+                           s->st));
     VK_HIP(hipMemcpyAsync(uhat, s->staging.p, bytes, device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s->st));
     VK_HIP(hipStreamSynchronize(s->st));
     return VK_OK;
@@ -626,7 +633,7 @@ int vk_solver_step_count(vk_solver *s, uint64_t *v) {
 }
 
 static int diag_sums(vk_solver *s, double *spec_out) {
-    VK_TRY(launch_diag(static_cast<const double *>(s->u.p), s->n, s->hp, s->S, s->ks, s->nshells,
+    VK_TRY(launch_diag(s->u.p, s->wp, s->n, s->hp, s->S, s->ks, s->nshells,
                        static_cast<double *>(s->partials.p), s->nblocks_diag, spec_out, s->dsc(), s->st));
     return 0;
 }
@@ -708,7 +715,7 @@ int vk_solver_nonlinear(vk_solver *s, const double *uhat, double *nhat) {
     if (!uhat || !nhat) return fail(VK_ERR_ARG, "null pointer");
     const size_t bytes = (size_t)3 * s->n * s->n * s->h * 16;
     VK_HIP(hipMemcpyAsync(s->staging.p, uhat, bytes, hipMemcpyHostToDevice, s->st));
-    VK_TRY(launch_repack(s->staging.p, s->x.p, (long long)3 * s->n * s->n, s->h, s->hp, s->h, 1, s->st));
+    VK_TRY(launch_repack_p(s->staging.p, F64, s->x.p, s->wp, (long long)3 * s->n * s->n, s->h, s->hp, s->h, 1, s->st));
     StageArgs sa = stage_args(s, ST_RAW, s->n1.p);
     if (int r = nl_eval(s, s->x.p, sa, nullptr, nullptr)) return r;
     return get_state_impl(s, nhat, false, s->n1.p);
@@ -768,16 +775,16 @@ void *woff(void *base, long long elems, size_t elem_bytes) {
 int ensure_vjp(vk_solver *s) {
     if (s->vjp_ready) return 0;
     const long long N = s->n, M = s->m, Hp = s->hp;
-    const size_t state_bytes = (size_t)s->S * 3 * 16;
+    const size_t state_bytes = s->state_bytes(), cot_bytes = (size_t)s->S * 3 * 16;
     const size_t wa = (size_t)std::max(8 * N * M * Hp, 6 * M * M * Hp), wb = (size_t)9 * M * M * Hp;
     int r = 0;
     if ((r = dev_alloc(s->sa_, state_bytes, s->total)) || (r = dev_alloc(s->sb_, state_bytes, s->total)) ||
-        (r = dev_alloc(s->sc_, state_bytes, s->total)) || (r = dev_alloc(s->lam, state_bytes, s->total)) ||
-        (r = dev_alloc(s->gu, state_bytes, s->total)) || (r = dev_alloc(s->g1p, state_bytes, s->total)) ||
-        (r = dev_alloc(s->cot, state_bytes, s->total)) || (r = dev_alloc(s->WA, wa * s->wbytes(), s->total)) ||
+        (r = dev_alloc(s->sc_, state_bytes, s->total)) || (r = dev_alloc(s->lam, cot_bytes, s->total)) ||
+        (r = dev_alloc(s->gu, cot_bytes, s->total)) || (r = dev_alloc(s->g1p, cot_bytes, s->total)) ||
+        (r = dev_alloc(s->cot, cot_bytes, s->total)) || (r = dev_alloc(s->WA, wa * s->wbytes(), s->total)) ||
         (r = dev_alloc(s->WB, wb * s->wbytes(), s->total)))
         return r;
-    VK_HIP(hipMemsetAsync(s->lam.p, 0, state_bytes, s->st));
+    VK_HIP(hipMemsetAsync(s->lam.p, 0, cot_bytes, s->st));
     s->vjp_ready = true;
     return 0;
 }
@@ -803,7 +810,7 @@ int vjp_nl(vk_solver *s, const void *v, StageArgs &sa) {
     const double nt = (double)N * N * N;
     // x inverse: u branch (5 fields) and cotangent branch (3 fields) -> WA
     Strided g1 = make_strided((int)NH, Hp, H, (int)NH, 0, NH, s->S, Hp, MH, Hp, NMH);
-    VK_TRY(launch_inv(INV_U3DX, M, N, F64, s->wp, v, s->WA.p, s->tw_m.p, g1, N, s->ks, 1.0, s->st));
+    VK_TRY(launch_inv(INV_U3DX, M, N, s->wp, s->wp, v, s->WA.p, s->tw_m.p, g1, N, s->ks, 1.0, s->st));
     VK_TRY(launch_inv(INV_U3, M, N, F64, s->wp, s->cot.p, woff(s->WA.p, 5 * NMH, wb), s->tw_m.p, g1, N, s->ks, 1.0,
                       s->st));
     // y inverse: u, w (6 fields) and g (3 fields) -> WB
@@ -849,10 +856,11 @@ int step_vjp(vk_solver *s) {
     return 0;
 }
 
-int upload_unpadded(vk_solver *s, const double *src, void *dst, bool device) {
+// unpadded complex double (host or device) -> a state-layout array of precision dp
+int upload_unpadded(vk_solver *s, const double *src, void *dst, bool device, Prec dp = F64) {
     const size_t bytes = (size_t)3 * s->n * s->n * s->h * 16;
     VK_HIP(hipMemcpyAsync(s->staging.p, src, bytes, device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s->st));
-    VK_TRY(launch_repack(s->staging.p, dst, (long long)3 * s->n * s->n, s->h, s->hp, s->h, 1, s->st));
+    VK_TRY(launch_repack_p(s->staging.p, F64, dst, dp, (long long)3 * s->n * s->n, s->h, s->hp, s->h, 1, s->st));
     return 0;
 }
 
@@ -896,7 +904,7 @@ struct Adjoint {
     long long forward_steps = 0;
     double nt2;
     int copy_state(void *dst, const void *src) {
-        VK_HIP(hipMemcpyAsync(dst, src, (size_t)s->S * 3 * 16, hipMemcpyDeviceToDevice, s->st));
+        VK_HIP(hipMemcpyAsync(dst, src, s->state_bytes(), hipMemcpyDeviceToDevice, s->st));
         return 0;
     }
     int advance(int k) {
@@ -907,7 +915,7 @@ struct Adjoint {
         return 0;
     }
     double enstrophy() {
-        if (launch_diag(static_cast<const double *>(s->u.p), s->n, s->hp, s->S, s->ks, s->nshells,
+        if (launch_diag(s->u.p, s->wp, s->n, s->hp, s->S, s->ks, s->nshells,
                         static_cast<double *>(s->partials.p), s->nblocks_diag, nullptr, s->dsc(), s->st))
             return NAN;
         Scalars h;
@@ -927,13 +935,13 @@ struct Adjoint {
             if (final_state) {
                 J += enstrophy();
                 if (!lamT_given)
-                    VK_TRY(launch_add_curlcurl(static_cast<double *>(s->lam.p), static_cast<const double *>(s->u.p),
+                    VK_TRY(launch_add_curlcurl(static_cast<double *>(s->lam.p), s->u.p, s->wp,
                                                s->n, s->hp, s->S, s->ks, 2.0 / nt2, s->st));
             }
         } else if (objective == VK_OBJ_DISSIPATION) {
             if (n >= 1) {
                 J += dt * 2.0 * nu * enstrophy();
-                VK_TRY(launch_add_curlcurl(static_cast<double *>(s->lam.p), static_cast<const double *>(s->u.p), s->n,
+                VK_TRY(launch_add_curlcurl(static_cast<double *>(s->lam.p), s->u.p, s->wp, s->n,
                                            s->hp, s->S, s->ks, 4.0 * nu * dt / nt2, s->st));
             }
         }
@@ -979,7 +987,7 @@ int adjoint_run(vk_solver *s, int nsteps, int objective, const double *lamT, int
     if (int r = ensure_vjp(s)) return r;
     while ((int)s->snaps.size() < S) {
         DevBuf b;
-        if (int r = dev_alloc(b, (size_t)s->S * 3 * 16, s->total)) return r;
+        if (int r = dev_alloc(b, s->state_bytes(), s->total)) return r;
         s->snaps.push_back(b);
     }
     build_schedule(s, nsteps, S);
@@ -1021,14 +1029,14 @@ int vk_solver_nonlinear_vjp(vk_solver *s, const double *uhat, const double *lam,
     if (int r = check(s)) return r;
     if (!uhat || !lam || !g) return fail(VK_ERR_ARG, "null pointer");
     if (int r = ensure_vjp(s)) return r;
-    if (int r = upload_unpadded(s, uhat, s->x.p, false)) return r;
+    if (int r = upload_unpadded(s, uhat, s->x.p, false, s->wp)) return r;
     if (int r = upload_unpadded(s, lam, s->lam.p, false)) return r;
-    StageArgs a = adj_args(s, AS_RAW, s->n1.p);
+    StageArgs a = adj_args(s, AS_RAW, s->gu.p);  // the result is complex double in both precisions
     VK_TRY(launch_adj_prep(static_cast<const double *>(s->lam.p), static_cast<double *>(s->cot.p), s->n, s->hp, s->S,
                            s->ks, a.inv_scale, static_cast<const double *>(s->tab.p), s->qmax + 1, s->dsc(), 0,
                            s->st));
     if (int r = vjp_nl(s, s->x.p, a)) return r;
-    return get_state_impl(s, g, false, s->n1.p);
+    return get_state_impl(s, g, false, s->gu.p, F64);
 }
 
 static int step_vjp_impl(vk_solver *s, const double *lam_out, double *lam_in, bool device) {
@@ -1037,7 +1045,7 @@ static int step_vjp_impl(vk_solver *s, const double *lam_out, double *lam_in, bo
     if (int r = ensure_vjp(s)) return r;
     if (int r = upload_unpadded(s, lam_out, s->lam.p, device)) return r;
     if (int r = step_vjp(s)) return r;
-    return get_state_impl(s, lam_in, device, s->lam.p);
+    return get_state_impl(s, lam_in, device, s->lam.p, F64);
 }
 int vk_solver_step_vjp(vk_solver *s, const double *lam_out, double *lam_in) {
     return step_vjp_impl(s, lam_out, lam_in, false);

@@ -66,23 +66,39 @@ static int last_launch() { return hipGetLastError() == hipSuccess ? 0 : 3; }
 // row repack: [nfields][rows][src_pitch] -> [nfields][rows][dst_pitch]
 // (complex double), copying `width` entries and zeroing the rest of each row
 // ---------------------------------------------------------------------------
-__global__ void k_repack(const cpx<double> *__restrict__ src, cpx<double> *__restrict__ dst, long long rows,
-                         int sp, int dp, int width, int nf) {
+template <typename TI, typename TO>
+__global__ void k_repack(const cpx<TI> *__restrict__ src, cpx<TO> *__restrict__ dst, long long rows, int sp, int dp,
+                         int width, int nf) {
     const long long total = (long long)nf * rows * dp;
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
          i += (long long)gridDim.x * blockDim.x) {
         const int k = (int)(i % dp);
         const long long r = i / dp;  // field*rows + row
-        dst[i] = k < width ? src[r * sp + k] : cpx<double>{0.0, 0.0};
+        if (k < width) {
+            const cpx<TI> v = src[r * sp + k];
+            dst[i] = {TO(v.re), TO(v.im)};
+        } else {
+            dst[i] = {TO(0), TO(0)};
+        }
     }
+}
+int launch_repack_p(const void *src, Prec sprec, void *dst, Prec dprec, long long rows, int src_pitch, int dst_pitch,
+                    int width, int nfields, hipStream_t st) {
+    const long long total = (long long)nfields * rows * dst_pitch;
+    const unsigned nb = (unsigned)std::min<long long>((total + 255) / 256, 65536);
+#define VK_REPACK(TI, TO)                                                                                          \
+    hipLaunchKernelGGL((k_repack<TI, TO>), dim3(nb), dim3(256), 0, st, reinterpret_cast<const cpx<TI> *>(src),   \
+                       reinterpret_cast<cpx<TO> *>(dst), rows, src_pitch, dst_pitch, width, nfields)
+    if (sprec == F64 && dprec == F64) VK_REPACK(double, double);
+    else if (sprec == F64) VK_REPACK(double, float);
+    else if (dprec == F64) VK_REPACK(float, double);
+    else VK_REPACK(float, float);
+#undef VK_REPACK
+    return last_launch();
 }
 int launch_repack(const void *src, void *dst, long long rows, int src_pitch, int dst_pitch, int width,
                   int nfields, hipStream_t st) {
-    const long long total = (long long)nfields * rows * dst_pitch;
-    const unsigned nb = (unsigned)std::min<long long>((total + 255) / 256, 65536);
-    hipLaunchKernelGGL(k_repack, dim3(nb), dim3(256), 0, st, reinterpret_cast<const cpx<double> *>(src),
-                       reinterpret_cast<cpx<double> *>(dst), rows, src_pitch, dst_pitch, width, nfields);
-    return last_launch();
+    return launch_repack_p(src, F64, dst, F64, rows, src_pitch, dst_pitch, width, nfields, st);
 }
 
 __global__ void k_fill_u64(unsigned long long *p, unsigned long long v, long long n) {
@@ -225,7 +241,8 @@ __device__ __forceinline__ void curl_mode(double kx, double ky, double kz, const
     w[2] = {-(kx * u[1].im - ky * u[0].im), kx * u[1].re - ky * u[0].re};
 }
 
-__global__ __launch_bounds__(DIAG_NT) void k_diag(const cpx<double> *__restrict__ u, int n, int hp, long long fs,
+template <typename TS>
+__global__ __launch_bounds__(DIAG_NT) void k_diag(const cpx<TS> *__restrict__ u, int n, int hp, long long fs,
                                                  double ks, int nshells, double *__restrict__ partials) {
     __shared__ double spec[DIAG_MAXSH];
     __shared__ double red[4][DIAG_NT];
@@ -241,7 +258,7 @@ __global__ __launch_bounds__(DIAG_NT) void k_diag(const cpx<double> *__restrict_
         const long long r = i / hp;
         const int iy = (int)(r % n), ix = (int)(r / n);
         const double kx = double(wavenum(ix, n)) * ks, ky = double(wavenum(iy, n)) * ks, kz = double(iz) * ks;
-        const cpx<double> uu[3] = {u[i], u[fs + i], u[2 * fs + i]};
+        const cpx<double> uu[3] = {cvt<double>(u[i]), cvt<double>(u[fs + i]), cvt<double>(u[2 * fs + i])};
         cpx<double> w[3], cw[3];
         curl_mode(kx, ky, kz, uu, w);
         curl_mode(kx, ky, kz, w, cw);
@@ -284,11 +301,15 @@ __global__ void k_diag_final(const double *__restrict__ partials, int nblocks, i
     }
 }
 
-int launch_diag(const double *u, int n, int hp, long long fs, double ks, int nshells, double *partials, int nblocks,
-                double *spec_out, Scalars *sc, hipStream_t st) {
+int launch_diag(const void *u, Prec sp, int n, int hp, long long fs, double ks, int nshells, double *partials,
+                int nblocks, double *spec_out, Scalars *sc, hipStream_t st) {
     if (nshells > DIAG_MAXSH) return 2;
-    hipLaunchKernelGGL(k_diag, dim3(nblocks), dim3(DIAG_NT), 0, st, reinterpret_cast<const cpx<double> *>(u), n, hp,
-                       fs, ks, nshells, partials);
+    if (sp == F64)
+        hipLaunchKernelGGL((k_diag<double>), dim3(nblocks), dim3(DIAG_NT), 0, st,
+                           reinterpret_cast<const cpx<double> *>(u), n, hp, fs, ks, nshells, partials);
+    else
+        hipLaunchKernelGGL((k_diag<float>), dim3(nblocks), dim3(DIAG_NT), 0, st,
+                           reinterpret_cast<const cpx<float> *>(u), n, hp, fs, ks, nshells, partials);
     const double nt = (double)n * n * n;
     hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(256), 0, st, partials, nblocks, nshells, 1.0 / (nt * nt), spec_out,
                        sc);
@@ -350,7 +371,8 @@ int launch_adj_prep(const double *lam, double *cot, int n, int hp, long long fs,
 }
 
 // lam += coef * w * curl(curl u) = coef * w * (k^2 u - k (k.u)), w the R2C weight
-__global__ void k_add_curlcurl(cpx<double> *__restrict__ lam, const cpx<double> *__restrict__ u, int n, int hp,
+template <typename TS>
+__global__ void k_add_curlcurl(cpx<double> *__restrict__ lam, const cpx<TS> *__restrict__ u, int n, int hp,
                                long long fs, double ks, double coef) {
     const long long total = (long long)n * n * hp;
     const int h = n / 2 + 1;
@@ -361,7 +383,7 @@ __global__ void k_add_curlcurl(cpx<double> *__restrict__ lam, const cpx<double> 
         mode_k(i, n, hp, ks, ix, iy, iz, kx, ky, kz);
         if (iz >= h) continue;
         const double w = (iz == 0 || iz == n / 2) ? 1.0 : 2.0;
-        const cpx<double> a = u[i], b = u[fs + i], c = u[2 * fs + i];
+        const cpx<double> a = cvt<double>(u[i]), b = cvt<double>(u[fs + i]), c = cvt<double>(u[2 * fs + i]);
         // curl(curl u) with the curl of spectral_ops.rs:93-104 applied twice
         const cpx<double> w0 = {-(ky * c.im - kz * b.im), ky * c.re - kz * b.re};
         const cpx<double> w1 = {-(kz * a.im - kx * c.im), kz * a.re - kx * c.re};
@@ -375,12 +397,16 @@ __global__ void k_add_curlcurl(cpx<double> *__restrict__ lam, const cpx<double> 
         lam[2 * fs + i] = {lam[2 * fs + i].re + s * c2.re, lam[2 * fs + i].im + s * c2.im};
     }
 }
-int launch_add_curlcurl(double *lam, const double *u, int n, int hp, long long fs, double ks, double coef,
+int launch_add_curlcurl(double *lam, const void *u, Prec sp, int n, int hp, long long fs, double ks, double coef,
                         hipStream_t st) {
     const long long total = (long long)n * n * hp;
     const unsigned nb = (unsigned)std::min<long long>((total + 255) / 256, 65536);
-    hipLaunchKernelGGL(k_add_curlcurl, dim3(nb), dim3(256), 0, st, reinterpret_cast<cpx<double> *>(lam),
-                       reinterpret_cast<const cpx<double> *>(u), n, hp, fs, ks, coef);
+    if (sp == F64)
+        hipLaunchKernelGGL((k_add_curlcurl<double>), dim3(nb), dim3(256), 0, st, reinterpret_cast<cpx<double> *>(lam),
+                           reinterpret_cast<const cpx<double> *>(u), n, hp, fs, ks, coef);
+    else
+        hipLaunchKernelGGL((k_add_curlcurl<float>), dim3(nb), dim3(256), 0, st, reinterpret_cast<cpx<double> *>(lam),
+                           reinterpret_cast<const cpx<float> *>(u), n, hp, fs, ks, coef);
     return last_launch();
 }
 

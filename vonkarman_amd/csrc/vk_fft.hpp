@@ -41,6 +41,11 @@ struct alignas(2 * sizeof(R)) cpx {
     R re, im;
 };
 
+template <typename TO, typename TI>
+VK_HD cpx<TO> cvt(cpx<TI> a) {
+    return {TO(a.re), TO(a.im)};
+}
+
 template <typename R>
 VK_HD cpx<R> cadd(cpx<R> a, cpx<R> b) { return {a.re + b.re, a.im + b.im}; }
 template <typename R>

@@ -16,10 +16,30 @@
 
 namespace vk {
 
-template <typename TO, typename TI>
-__device__ __forceinline__ cpx<TO> cvt(cpx<TI> a) {
-    return {TO(a.re), TO(a.im)};
-}
+
+// Minimum waves per SIMD asked of the compiler for each kernel class and work
+// type (it caps the VGPRs per thread: 512 / waves).
+enum KernelClass { K_INV, K_FWD, K_ZNL, K_ZVJP };
+#ifndef VK_WAVES_INV_F32
+#define VK_WAVES_INV_F32 1
+#endif
+#ifndef VK_WAVES_FWD_F32
+#define VK_WAVES_FWD_F32 1
+#endif
+#ifndef VK_WAVES_ZNL_F32
+#define VK_WAVES_ZNL_F32 4  // 128 VGPRs: 4 blocks of 4 waves per CU (1.47x faster than uncapped)
+#endif
+#ifndef VK_WAVES_ZNL_F64
+#define VK_WAVES_ZNL_F64 1
+#endif
+template <int K, typename TW>
+struct Waves {
+    static constexpr int value = sizeof(TW) == 8 ? (K == K_ZNL ? VK_WAVES_ZNL_F64 : 1)
+                                 : K == K_INV  ? VK_WAVES_INV_F32
+                                 : K == K_FWD  ? VK_WAVES_FWD_F32
+                                 : K == K_ZNL  ? VK_WAVES_ZNL_F32
+                                               : 1;
+};
 
 __device__ __forceinline__ long long col_addr(int col, int inner, long long os) {
     return (long long)(col / inner) * os + (col % inner);
@@ -50,7 +70,7 @@ template <int L, int MODE, typename TW>
 using InvGeom = Geom<L, InvPlan<L, MODE>::type::E, TW>;
 
 template <int L, int LIN, int MODE, typename TI, typename TW>
-__global__ __launch_bounds__((InvGeom<L, MODE, TW>::NT)) void k_inv(const cpx<TI> *__restrict__ in,
+__global__ __launch_bounds__((InvGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_waves_per_eu(Waves<K_INV, TW>::value))) void k_inv(const cpx<TI> *__restrict__ in,
                                                               cpx<TW> *__restrict__ out,
                                                               const cpx<TW> *__restrict__ tw, Strided g, int n,
                                                               double ks, TW scale) {
@@ -293,7 +313,7 @@ template <int L, int MODE, typename TW>
 using FwdGeom = Geom<L, FwdPlan<L, MODE>::type::E, TW>;
 
 template <int L, int LOUT, int MODE, int NF, typename TW>
-__global__ __launch_bounds__((FwdGeom<L, MODE, TW>::NT)) void k_fwd(const cpx<TW> *__restrict__ in,
+__global__ __launch_bounds__((FwdGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_waves_per_eu(Waves<K_FWD, TW>::value))) void k_fwd(const cpx<TW> *__restrict__ in,
                                                                     cpx<TW> *__restrict__ out,
                                                                     const cpx<TW> *__restrict__ tw, Strided g,
                                                                     int n, double ks, TW scale, StageArgs sa) {
@@ -444,9 +464,9 @@ __global__ __launch_bounds__((FwdGeom<L, MODE, TW>::NT)) void k_fwd(const cpx<TW
                     y = {y.re - ky * dr * inv, y.im - ky * di * inv};
                     z = {z.re - kz * dr * inv, z.im - kz * di * inv};
                 }
-                const cpx<double> nn[3] = {cvt<double>(x), cvt<double>(y), cvt<double>(z)};
+                const cpx<TW> nn[3] = {x, y, z};
                 const long long idx = cout + (long long)j * g.out_ls;
-                stage_epilogue<double>(sa, idx, wx * wx + wy * wy + iz * iz, nn);
+                stage_epilogue<TW>(sa, idx, wx * wx + wy * wy + iz * iz, nn);
             }
     }
 }
@@ -482,15 +502,19 @@ __device__ __forceinline__ cpx<TW> z_pair_at(gptr<cpx<TW>> fa, gptr<cpx<TW>> fb,
 }
 
 // z tiles are column-interleaved too: the T lines of a block are the columns,
-// so position p of all T lines is one contiguous LDS row (no bank conflicts)
-// and each line still reads HBM in 128-byte runs across a wavefront.
+// so position p of all T lines is one contiguous LDS row.  64-byte rows (small
+// blocks) let more blocks share a CU; each line still reads HBM in runs of
+// 16 consecutive positions across a wavefront.
+#ifndef VK_Z_ROW_BYTES
+#define VK_Z_ROW_BYTES 64  // T = 4 (FP64) or 8 (FP32) lines per block
+#endif
 template <int L, typename TW>
 using ZNLP = PlanE12<L>;
 template <int L, typename TW>
-using ZNLGeom = Geom<L, ZNLP<L, TW>::E, TW>;
+using ZNLGeom = Geom<L, ZNLP<L, TW>::E, TW, VK_Z_ROW_BYTES>;
 
 template <int L, typename TW>
-__global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) void k_z_nl(const cpx<TW> *__restrict__ in,
+__global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) __attribute__((amdgpu_waves_per_eu(Waves<K_ZNL, TW>::value))) void k_z_nl(const cpx<TW> *__restrict__ in,
                                                                cpx<TW> *__restrict__ out, long long lines, int hp,
                                                                long long fs_in, long long fs_out, TW s_in, TW s_out,
                                                                const cpx<TW> *__restrict__ tw) {
@@ -604,7 +628,7 @@ __device__ __forceinline__ void z_split_store(const cpx<TW> (&v)[E], cpx<TW> *ld
 // pass reads them) and 6..8 = g, the cotangent of the truncated R2C output.
 // out: 0..2 = GU = C2R^T (w x g), 3..5 = GW = C2R^T (g x u), truncated.
 template <int L, typename TW>
-__global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) void k_z_vjp(const cpx<TW> *__restrict__ in,
+__global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) __attribute__((amdgpu_waves_per_eu(Waves<K_ZVJP, TW>::value))) void k_z_vjp(const cpx<TW> *__restrict__ in,
                                                                 cpx<TW> *__restrict__ out, long long lines, int hp,
                                                                 long long fs_in, long long fs_out, TW s_u, TW s_o,
                                                                 const cpx<TW> *__restrict__ tw) {
@@ -876,6 +900,8 @@ int launch_inv_L(int mode, int lin, Prec ip, Prec wp, const void *in, void *out,
             return run_inv<L, LIN, INV_U3, double, float>(in, out, tw, g, n, ks, scale, st);
         if (ip == F32 && wp == F32 && mode == INV_UOMEGA)
             return run_inv<L, LIN, INV_UOMEGA, float, float>(in, out, tw, g, n, ks, scale, st);
+        if (ip == F32 && wp == F32 && mode == INV_U3DX)
+            return run_inv<L, LIN, INV_U3DX, float, float>(in, out, tw, g, n, ks, scale, st);
         if (ip == F32 && wp == F32 && mode == INV_U3)
             return run_inv<L, LIN, INV_U3, float, float>(in, out, tw, g, n, ks, scale, st);
     }

@@ -60,6 +60,8 @@ enum Stage {
 };
 struct StageArgs {
     int stage;
+    // state-layout arrays (out, u, n1, sacc) are in the work precision: complex
+    // double for FP64 solvers, complex float for mixed ones
     void *out;        // ST_RAW: N hat; ST_1..3: next stage input; ST_4: u (in place)
     const void *u;    // state
     void *n1;         // N1 (written by ST_1, read by ST_3/ST_4)
@@ -68,7 +70,7 @@ struct StageArgs {
     const double *tab;  // ETD table, 7 arrays of (qmax+1) doubles
     int tab_stride;     // qmax+1
     const Scalars *sc;  // dt
-    // reverse sweep (AS_*): lam = cotangent of the step output, gu = cotangent
+    // reverse sweep (AS_*, complex double in both precisions): lam = cotangent of the step output, gu = cotangent
     // accumulated on u, g1p = partial cotangent of N1, cot = next VJP input
     // (projected and scaled), out = cotangent of the step input (AS_1)
     const void *lam;
@@ -103,20 +105,23 @@ int launch_adj_prep(const double *lam, double *cot, int n, int hp, long long fs,
                     const double *tab, int tab_stride, const Scalars *sc, int use_b4, hipStream_t st);
 // lam += coef * w * curl(curl(u)) (w = R2C weight), state layout: the gradient
 // of coef/2 * sum w |curl u|^2
-int launch_add_curlcurl(double *lam, const double *u, int n, int hp, long long fs, double ks, double coef,
+int launch_add_curlcurl(double *lam, const void *u, Prec sp, int n, int hp, long long fs, double ks, double coef,
                         hipStream_t st);
 
 // helpers ---------------------------------------------------------------------
 int launch_repack(const void *src, void *dst, long long rows, int src_pitch, int dst_pitch,
                   int width, int nfields, hipStream_t st);
+// repack with a precision change (complex double <-> complex float)
+int launch_repack_p(const void *src, Prec sprec, void *dst, Prec dprec, long long rows, int src_pitch, int dst_pitch,
+                    int width, int nfields, hipStream_t st);
 int launch_fill_u64(unsigned long long *p, unsigned long long v, long long n, hipStream_t st);
 int launch_cfl_update(Scalars *sc, double dx, double nu, double safety, int unconditional,
                       hipStream_t st);
 int launch_clock_advance(Scalars *sc, hipStream_t st);
 int launch_etd_table(double *tab, int qmax, double nu, double ks, const Scalars *sc,
                      int force, hipStream_t st);
-// diagnostics: E, Omega, H, S and the shell spectrum in one pass over u
-int launch_diag(const double *u, int n, int hp, long long fs, double ks, int nshells,
+// diagnostics: E, Omega, H, S and the shell spectrum in one pass over u (state precision sp)
+int launch_diag(const void *u, Prec sp, int n, int hp, long long fs, double ks, int nshells,
                 double *partials, int nblocks, double *spec_out, Scalars *sc, hipStream_t st);
 
 }  // namespace vk
