@@ -413,8 +413,9 @@ KERNEL_OF_PASS = {  # kernel-name prefixes of the FP64 passes at the padded leng
     "x_inverse": "k_inv<{M}, {N}, 2, double, double>",
     "y_inverse": "k_inv<{M}, {N}, 4, double, double>",
     "z_cross": "k_z_nl<{M}, double>",
-    "y_forward": "k_fwd<{M}, {N}, 0, 3, double>",
-    "x_forward": "k_fwd<{M}, {N}, 1, 3, double>",
+    "y_forward": "k_fwd<{M}, {N}, 0, 3, 0, double>",
+    "x_forward": "k_fwd<{M}, {N}, 1, 3, ",  # the four stage instances
+
 }
 
 
@@ -429,22 +430,23 @@ def pmc_child(n: int) -> None:
     s.steps(1)
 
 
-def measure_traffic(n: int, pass_name: str, timeout: float = 150.0):
+def measure_traffic(n: int, pass_name: str, timeout: float = 150.0) -> dict:
     """HBM bytes per launch of one pass from two rocprofv3 --pmc runs
     (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch).  FETCH_SIZE is scaled by
     the ratio of known bytes to the counter's reading for the state repack
     kernel of the same run (a plain copy that reads 3 n^2 rows of n/2+1
     complex doubles), WRITE_SIZE by the same ratio for its write (3 n^2 rows
-    of the padded row pitch).  Returns None if the profiler is not
-    available."""
+    of the padded row pitch).  On failure returns {"error": reason}."""
     import re
     import shutil
     import sqlite3
     import subprocess
     import tempfile
 
-    if not shutil.which("rocprofv3"):
-        return None
+    prof = shutil.which("rocprofv3") or ("/opt/rocm/bin/rocprofv3" if os.path.exists("/opt/rocm/bin/rocprofv3")
+                                         else None)
+    if not prof:
+        return {"error": "rocprofv3 not found"}
     M, N, H = 3 * n // 2, n, n // 2 + 1
     want = KERNEL_OF_PASS[pass_name].format(M=M, N=N)
     known_read_kib = 3 * n * n * H * 16 / 1024.0
@@ -452,23 +454,39 @@ def measure_traffic(n: int, pass_name: str, timeout: float = 150.0):
     vals = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         d = tempfile.mkdtemp(prefix="vk_pmc_", dir=os.environ.get("TMPDIR", "/tmp"))
-        cmd = ["rocprofv3", "--pmc", counter, "-d", d, "-o", "run", "--", sys.executable,
+        cmd = [prof, "--pmc", counter, "-d", d, "-o", "run", "--", sys.executable,
                os.path.abspath(__file__), "--pmc-child", "--n", str(n)]
         try:
-            subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=timeout, check=False)
+            cp = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, timeout=timeout, check=False)
         except subprocess.TimeoutExpired:
-            return None
+            return {"error": f"{counter} run timed out after {timeout} s"}
         dbs = [os.path.join(r, f) for r, _, fs in os.walk(d) for f in fs if f.endswith("_results.db")]
-        if not dbs:
-            return None
-        con = sqlite3.connect(dbs[0])
-        rows = list(con.execute("select kernel_name, value from counters_collection order by dispatch_id"))
+        csvs = [os.path.join(r, f) for r, _, fs in os.walk(d) for f in fs if f.endswith("counter_collection.csv")]
+        rows = []
+        if dbs:
+            con = sqlite3.connect(dbs[0])
+            tables = [t for (t,) in con.execute("select name from sqlite_master where type in ('table','view')")]
+            tab = next((t for t in tables if t == "counters_collection"), None)
+            if tab is None:
+                tail = cp.stdout.decode(errors="replace")[-300:]
+                return {"error": f"no counters_collection table in {os.path.basename(dbs[0])} "
+                                 f"(tables: {tables[:12]}); rc={cp.returncode}; tail: {tail}"}
+            rows = list(con.execute(f"select kernel_name, value from {tab} order by dispatch_id"))
+        elif csvs:
+            import csv
+
+            with open(csvs[0]) as fh:
+                rows = [(r["Kernel_Name"], float(r["Counter_Value"])) for r in csv.DictReader(fh)]
+        else:
+            tail = cp.stdout.decode(errors="replace")[-400:]
+            return {"error": f"{counter}: no results (rc={cp.returncode}): {tail}"}
         short = [(re.sub(r"\(.*", "", k).replace("void ", "").replace("vk::", ""), v) for k, v in rows]
-        rep = [v for k, v in short if k == "k_repack"]
+        rep = [v for k, v in short if k.startswith("k_repack")]
         ker = [v for k, v in short if k.startswith(want)]
         shutil.rmtree(d, ignore_errors=True)
         if not rep or not ker:
-            return None
+            names = sorted({k for k, _ in short})[:10]
+            return {"error": f"{counter}: kernel rows missing (repack {len(rep)}, {want} {len(ker)}); seen {names}"}
         vals[counter] = (rep[0], sum(ker) / len(ker))
     fetch_scale = known_read_kib / vals["FETCH_SIZE"][0]
     write_scale = known_write_kib / vals["WRITE_SIZE"][0]
@@ -555,7 +573,7 @@ def main():
                                "adaptive CFL 0.5, E(k) every 10 steps", "grid": n, "padded": 3 * n // 2,
                    "nu": 1e-3, "cfl_safety": 0.5, "spectrum_interval": 10},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None if traffic is None else traffic["bytes"],
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": (traffic or {}).get("bytes"),
                      "bytes_per_launch": pb[dom], "avg_launch_ms": avg[dom],
                      "traffic_detail": traffic},
         "cpu_baseline": cpu,

@@ -311,6 +311,43 @@ VK_HD int ci_pos_out(int tid, int e) {
     static_assert(NT % T == 0, "threads must cover whole rows");
     return tid / T + (e / RP) * (NT / T) + (e % RP) * (L / RP);
 }
+// The positions of element e over all threads of a column-interleaved tile
+// form the contiguous range [ci_in_base(e), ci_in_base(e) + NT/T): the part
+// that depends on the thread is tid / T.  Loaders classify each element's
+// range at compile time (all in the zero gap, all stored, or mixed).
+template <int L, int R1, int T, int NT>
+constexpr int ci_in_base(int e) {
+    return (e / R1) * (NT / T) + (e % R1) * (L / R1);
+}
+
+// Positions of the (3/2-rule) zero gap [lin/2, L - lin/2) of a padded line
+// relative to the range [b, b + w): 0 = entirely outside the gap (every
+// position maps to a stored input), 1 = entirely inside (all zero), 2 = mixed.
+constexpr int gap_class(int b, int w, int lin, int L) {
+    if (lin == L) return 0;
+    const int g0 = lin / 2, g1 = L - lin / 2;  // gap is [g0, g1)
+    if (b >= g0 && b + w <= g1) return 1;
+    if (b + w <= g0 || b >= g1) return 0;
+    return 2;
+}
+
+// call f(std::integral_constant<int, i>) for i = 0..N-1 (compile-time indices,
+// so per-element properties of a tile can be decided by `if constexpr`)
+template <int I>
+struct IntC {
+    static constexpr int value = I;
+};
+template <int I, int N, typename F>
+VK_HD void static_for_from(F &&f) {
+    if constexpr (I < N) {
+        f(IntC<I>{});
+        static_for_from<I + 1, N>(f);
+    }
+}
+template <int N, typename F>
+VK_HD void static_for(F &&f) {
+    static_for_from<0, N>(f);
+}
 
 // ---------------------------------------------------------------------------
 // plans.  Each kernel class picks radices (first to last) and E, the number of
@@ -412,8 +449,10 @@ __device__ __forceinline__ cpx<R> ldg(gptr<cpx<R>> p, long long i) {
 }
 
 // one pass of the register-staged transform
-template <int L, int NS, int D, int T, int NT, int SP, int SC, int E, bool FIRST, typename R, int RAD, int... REST>
-__device__ __forceinline__ void fft_reg_run(cpx<R> (&v)[E], cpx<R> *lds, gptr<cpx<R>> tw, int tid) {
+// (TP: the twiddle table pointer, global or LDS)
+template <int L, int NS, int D, int T, int NT, int SP, int SC, int E, bool FIRST, typename R, typename TP, int RAD,
+          int... REST>
+__device__ __forceinline__ void fft_reg_run(cpx<R> (&v)[E], cpx<R> *lds, TP tw, int tid) {
     static_assert(E % RAD == 0, "E must be a multiple of every radix");
     constexpr int BPT = E / RAD;
     static_assert(T * (L / RAD) == BPT * NT, "butterflies must split evenly over the threads");
@@ -443,14 +482,13 @@ __device__ __forceinline__ void fft_reg_run(cpx<R> (&v)[E], cpx<R> *lds, gptr<cp
             for (int m = 0; m < RAD; ++m) lds[bfly_out<L, RAD, NS>(j, m) * SP + c * SC] = v[b * RAD + m];
         }
         __syncthreads();
-        fft_reg_run<L, NS * RAD, D, T, NT, SP, SC, E, false, R, REST...>(v, lds, tw, tid);
+        fft_reg_run<L, NS * RAD, D, T, NT, SP, SC, E, false, R, TP, REST...>(v, lds, tw, tid);
     }
 }
 
-template <int L, int D, int T, int NT, int SP, int SC, int E, typename R, int... RS>
-__device__ __forceinline__ void fft_reg_list(RadixList<RS...>, cpx<R> (&v)[E], cpx<R> *lds, gptr<cpx<R>> tw,
-                                             int tid) {
-    fft_reg_run<L, 1, D, T, NT, SP, SC, E, true, R, RS...>(v, lds, tw, tid);
+template <int L, int D, int T, int NT, int SP, int SC, int E, typename R, typename TP, int... RS>
+__device__ __forceinline__ void fft_reg_list(RadixList<RS...>, cpx<R> (&v)[E], cpx<R> *lds, TP tw, int tid) {
+    fft_reg_run<L, 1, D, T, NT, SP, SC, E, true, R, TP, RS...>(v, lds, tw, tid);
 }
 
 // Transform the registers v (filled in reg_pos_in order of the plan's first
@@ -464,6 +502,27 @@ __device__ __forceinline__ void fft_reg(cpx<R> (&v)[E], cpx<R> *lds, const cpx<R
     // out of the callers' per-field loops, which would hold them all in
     // registers at once.
     fft_reg_list<L, D, T, NT, SP, SC>(PL{}, v, lds, opaque(tw), tid);
+}
+
+// The same transform with the twiddle table in LDS (ltw, L entries, filled
+// by stage_twiddles).  No vector memory instruction runs inside the
+// transform, so HBM loads issued before it (a prefetch of the next field)
+// stay in flight until their data is used: the memory counter that a twiddle
+// load would wait on counts those loads too, in issue order.
+template <int L, int D, int T, int NT, int SP, int SC, typename PL, int E, typename R>
+__device__ __forceinline__ void fft_reg_lt(cpx<R> (&v)[E], cpx<R> *lds, const cpx<R> *ltw, int tid) {
+    static_assert(PL::product() == L, "plan must multiply to L");
+    fft_reg_list<L, D, T, NT, SP, SC>(PL{}, v, lds, ltw, tid);
+}
+
+// copy the twiddle table of length L into LDS; the caller synchronises
+// (the first transform's barrier comes after its first pass, which reads no
+// twiddles, so a __syncthreads() before the first transform is enough)
+template <int L, int NT, typename R>
+__device__ __forceinline__ void stage_twiddles(cpx<R> *ltw, const cpx<R> *__restrict__ tw, int tid) {
+    const gptr<cpx<R>> g = (gptr<cpx<R>>)tw;
+#pragma unroll
+    for (int i = tid; i < L; i += NT) ltw[i] = ldg(g, i);
 }
 #endif
 

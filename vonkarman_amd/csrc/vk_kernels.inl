@@ -19,7 +19,7 @@ namespace vk {
 
 // Minimum waves per SIMD asked of the compiler for each kernel class and work
 // type (it caps the VGPRs per thread: 512 / waves).
-enum KernelClass { K_INV, K_FWD, K_ZNL, K_ZVJP };
+enum KernelClass { K_INV, K_FWD, K_FVJP, K_ZNL, K_ZVJP };
 #ifndef VK_WAVES_INV_F32
 #define VK_WAVES_INV_F32 1
 #endif
@@ -32,9 +32,18 @@ enum KernelClass { K_INV, K_FWD, K_ZNL, K_ZVJP };
 #ifndef VK_WAVES_ZNL_F64
 #define VK_WAVES_ZNL_F64 1
 #endif
+#ifndef VK_WAVES_INV_F64
+#define VK_WAVES_INV_F64 1
+#endif
+#ifndef VK_WAVES_FWD_F64
+#define VK_WAVES_FWD_F64 2  // 256 VGPRs: keeps the stage-3/4 epilogues at 2 waves (no AGPR overflow)
+#endif
 template <int K, typename TW>
 struct Waves {
-    static constexpr int value = sizeof(TW) == 8 ? (K == K_ZNL ? VK_WAVES_ZNL_F64 : 1)
+    static constexpr int value = sizeof(TW) == 8 ? (K == K_ZNL   ? VK_WAVES_ZNL_F64
+                                                    : K == K_INV ? VK_WAVES_INV_F64
+                                                    : K == K_FWD ? VK_WAVES_FWD_F64
+                                                                 : 1)
                                  : K == K_INV  ? VK_WAVES_INV_F32
                                  : K == K_FWD  ? VK_WAVES_FWD_F32
                                  : K == K_ZNL  ? VK_WAVES_ZNL_F32
@@ -66,8 +75,15 @@ template <int L>
 struct InvPlan<L, INV_U3> {
     using type = PlanS<L>;
 };
+// bytes of each position row of a strided tile (T = row bytes / element bytes columns)
+#ifndef VK_ROW_BYTES_INV
+#define VK_ROW_BYTES_INV 128
+#endif
+#ifndef VK_ROW_BYTES_FWD
+#define VK_ROW_BYTES_FWD 128
+#endif
 template <int L, int MODE, typename TW>
-using InvGeom = Geom<L, InvPlan<L, MODE>::type::E, TW>;
+using InvGeom = Geom<L, InvPlan<L, MODE>::type::E, TW, VK_ROW_BYTES_INV>;
 
 template <int L, int LIN, int MODE, typename TI, typename TW>
 __global__ __launch_bounds__((InvGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_waves_per_eu(Waves<K_INV, TW>::value))) void k_inv(const cpx<TI> *__restrict__ in,
@@ -78,11 +94,16 @@ __global__ __launch_bounds__((InvGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
     using PL = typename P::radices;
     constexpr int E = P::E;
     constexpr int T = InvGeom<L, MODE, TW>::T, NT = InvGeom<L, MODE, TW>::NT;
+    // prefetch the next input field while transforming (multi-field modes with
+    // the 12-element plan: a 24-element prefetch buffer would cost occupancy)
+    constexpr bool PF = MODE != INV_PLAIN && E <= 12;
     __shared__ cpx<TW> lds[L * T];
+    __shared__ cpx<TW> ltw[PF ? L : 1];
     const int tid = threadIdx.x;
     const int col0 = blockIdx.x * T;
     const cpx<TW> zero = {TW(0), TW(0)};
     cpx<TW> v[E];
+    if constexpr (PF) stage_twiddles<L, NT>(ltw, tw, tid);
 
     // this thread's column (NT is a multiple of T) and its base offsets
     const int col = col0 + tid % T;
@@ -95,16 +116,36 @@ __global__ __launch_bounds__((InvGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
         const int p = ci_pos_in<L, PL::first, T, NT>(tid, e);
         return LIN == L ? p : unpad_index(p, LIN, L);
     };
-    cpx<TW> A[E], B[E];
+    // A, B: input fields being combined; Q: the next input field, loaded
+    // while the current one is transformed (the transforms take their
+    // twiddles from LDS, so no vector memory wait inside them holds for Q)
+    cpx<TW> A[E], B[E], Q[E];
     // fill a register slot with input field f (opaque pointer: the loads stay
-    // after the previous transform's barrier)
+    // where they are written, ahead of the transform they overlap).  Every
+    // load is unconditional, from a valid address (column 0 for columns
+    // outside the geometry, index 0 for positions in the zero gap), and zeros
+    // are selected afterwards: a load under a branch is waited for before the
+    // branch joins, which would serialise the loads.  Elements whose whole
+    // position range lies in the gap are zero at compile time and never loaded.
     auto fill = [&](cpx<TW>(&S)[E], int f) {
         const gptr<cpx<TI>> src = opaque(in) + f * g.in_fs + cin;
+        static_for<E>([&](auto ec) {
+            constexpr int e = decltype(ec)::value;
+            constexpr int gc = gap_class(ci_in_base<L, PL::first, T, NT>(e), NT / T, LIN, L);
+            if constexpr (gc == 1) {
+                S[e] = zero;
+            } else if constexpr (gc == 0) {
+                S[e] = cvt<TW>(ldg(src, (long long)uidx(e) * g.in_ls));
+            } else {
+                const int i = uidx(e);
+                const cpx<TW> x = cvt<TW>(ldg(src, (long long)(i < 0 ? 0 : i) * g.in_ls));
+                S[e] = i < 0 ? zero : x;
+            }
+        });
+    };
+    auto take = [&](cpx<TW>(&S)[E]) {
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const int i = uidx(e);
-            S[e] = (colok && i >= 0) ? cvt<TW>(ldg(src, (long long)i * g.in_ls)) : zero;
-        }
+        for (int e = 0; e < E; ++e) S[e] = Q[e];
     };
     auto kline = [&](int e) -> TW {
         const int i = uidx(e);
@@ -113,7 +154,8 @@ __global__ __launch_bounds__((InvGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
     auto emit = [&](int fo, auto value) {
 #pragma unroll
         for (int e = 0; e < E; ++e) v[e] = value(e);
-        fft_reg<L, +1, T, NT, T, 1, PL>(v, lds, tw, tid);
+        if constexpr (PF) fft_reg_lt<L, +1, T, NT, T, 1, PL>(v, lds, ltw, tid);
+        else fft_reg<L, +1, T, NT, T, 1, PL>(v, lds, tw, tid);
         if (colok) {
 #pragma unroll
             for (int e = 0; e < E; ++e) {
@@ -139,87 +181,168 @@ __global__ __launch_bounds__((InvGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
         return cpx<TW>{B[e].re + ky * A[e].im, B[e].im - ky * A[e].re};
     };
 
-    if constexpr (MODE == INV_PLAIN || MODE == INV_U3) {
-        constexpr int NF = MODE == INV_PLAIN ? 1 : 3;
+    if constexpr (MODE == INV_PLAIN) {
+        fill(A, 0);
+        emit(0, copyA);
+    } else if constexpr (MODE == INV_U3 && !PF) {
 #pragma unroll 1
-        for (int f = 0; f < NF; ++f) {
+        for (int f = 0; f < 3; ++f) {
             fill(A, f);
             emit(f, copyA);
         }
+    } else if constexpr (MODE == INV_U3) {
+        fill(A, 0);
+        fill(Q, 1);
+        emit(0, copyA);
+        take(A);
+        fill(Q, 2);
+        emit(1, copyA);
+        take(A);
+        emit(2, copyA);
     } else if constexpr (MODE == INV_U3DX) {
+        static_assert(PF, "the x pass prefetches");
         // outputs: 0 ux, 1 uy, 2 uz, 3 i kx uz, 4 i kx uy
         fill(A, 0);
+        fill(Q, 1);
         emit(0, copyA);
-        fill(A, 1);
+        take(A);
+        fill(Q, 2);
         emit(1, copyA);
         emit(4, [&](int e) { return ik(kline(e), A[e]); });
-        fill(A, 2);
+        take(A);
         emit(2, copyA);
         emit(3, [&](int e) { return ik(kline(e), A[e]); });
     } else {
-        // UOMEGA outputs 0..2 u, 3 wx, 4 wy, 5 wz; OMEGA outputs 0 wx, 1 wy, 2 wz
+        // UOMEGA outputs 0..2 u, 3 wx, 4 wy, 5 wz; OMEGA outputs 0 wx, 1 wy, 2 wz.
+        // wx is emitted before the plain uz copy: uy is dead once wx's input is
+        // formed, so no transform runs with two input fields live besides Q.
         constexpr int OW = (MODE == INV_UOMEGA) ? 3 : 0;
         fill(A, 0);
+        fill(Q, 3);
         if constexpr (MODE == INV_UOMEGA) emit(0, copyA);
-        fill(B, 3);
+        take(B);
+        fill(Q, 4);
         emit(OW + 1, wy);
-        fill(B, 4);
+        take(B);
+        fill(Q, 1);
         emit(OW + 2, wz);
-        fill(A, 1);
+        take(A);
+        fill(Q, 2);
         if constexpr (MODE == INV_UOMEGA) emit(1, copyA);
-        fill(B, 2);
-        if constexpr (MODE == INV_UOMEGA) emit(2, copyB);
+        take(B);
         emit(OW + 0, wx);
+        if constexpr (MODE == INV_UOMEGA) emit(2, copyB);
     }
 }
 
 // ===========================================================================
 // Forward strided pass: lines of L physical values, forward DFT, the LOUT kept
 // modes (3/2-rule truncation map) scaled and stored.  FWD_STAGE additionally
-// projects (Leray) and runs one ETD-RK4 stage epilogue per mode.
+// projects (Leray) and runs one ETD-RK4 stage epilogue per mode; FWD_VJP forms
+// t = GU + curl GW and runs one reverse-stage epilogue per mode.  The stage is
+// a template parameter, so each epilogue's loads are known at compile time and
+// are issued together per chunk of CH kept modes (the state arrays may alias
+// as far as the compiler knows, so loads of the next chunk wait behind the
+// stores of this one).  Loads are unconditional (columns outside the geometry
+// read column 0); only the stores are predicated.
 // ===========================================================================
-template <typename TS>
-__device__ __forceinline__ void stage_epilogue(const StageArgs &sa, long long idx, int q, const cpx<TS> nn[3]) {
+// ETD-RK4 stage epilogue (vonkarman-periodic/src/solver.rs:264-378) for CH
+// modes; nn = Leray-projected N-hat of the modes.  Table rows: 0 e, 1 e_half,
+// 2 a21 (= a31), 3 a41, 4 b1, 5 b23, 6 b4.
+template <int STG, int CH, typename TS>
+__device__ __forceinline__ void stage_chunk(const StageArgs &sa, bool st, const long long (&idx)[CH],
+                                            const int (&q)[CH], const cpx<TS> (&nn)[CH][3]) {
     cpx<TS> *out = reinterpret_cast<cpx<TS> *>(sa.out);
     const cpx<TS> *u = reinterpret_cast<const cpx<TS> *>(sa.u);
     cpx<TS> *n1 = reinterpret_cast<cpx<TS> *>(sa.n1);
     cpx<TS> *sacc = reinterpret_cast<cpx<TS> *>(sa.sacc);
     const double *tab = sa.tab;
     const int ts = sa.tab_stride;
-    const double dt = sa.sc->dt;
-    // table rows: 0 e, 1 e_half, 2 a21 (= a31), 3 a41, 4 b1, 5 b23, 6 b4
-    if (sa.stage == ST_RAW) {
-        for (int f = 0; f < 3; ++f) out[f * sa.fs + idx] = nn[f];
-    } else if (sa.stage == ST_1 || sa.stage == ST_2) {
-        const double eh = tab[1 * ts + q], a = tab[2 * ts + q];
-        for (int f = 0; f < 3; ++f) {
-            const cpx<TS> uu = u[f * sa.fs + idx];
-            const cpx<TS> x = nn[f];
-            if (sa.stage == ST_1) n1[f * sa.fs + idx] = x;
-            else sacc[f * sa.fs + idx] = x;
-            out[f * sa.fs + idx] = {TS(eh * uu.re + dt * a * x.re), TS(eh * uu.im + dt * a * x.im)};
+    const long long fs = sa.fs;
+    if constexpr (STG == ST_RAW) {
+        if (st) {
+#pragma unroll
+            for (int i = 0; i < CH; ++i)
+#pragma unroll
+                for (int f = 0; f < 3; ++f) out[f * fs + idx[i]] = nn[i][f];
         }
-    } else if (sa.stage == ST_3) {
-        const double e = tab[0 * ts + q], a = tab[3 * ts + q];
-        for (int f = 0; f < 3; ++f) {
-            const cpx<TS> uu = u[f * sa.fs + idx];
-            const cpx<TS> x = nn[f];
-            const cpx<TS> s2 = sacc[f * sa.fs + idx];
-            sacc[f * sa.fs + idx] = {s2.re + x.re, s2.im + x.im};
-            const cpx<TS> m1 = n1[f * sa.fs + idx];
-            const double dr = 2.0 * x.re - m1.re, di = 2.0 * x.im - m1.im;
-            out[f * sa.fs + idx] = {TS(e * uu.re + dt * a * dr), TS(e * uu.im + dt * a * di)};
+    } else if constexpr (STG == ST_1 || STG == ST_2) {
+        const double dt = sa.sc->dt;
+        double eh[CH], a[CH];
+        cpx<TS> uu[CH][3];
+#pragma unroll
+        for (int i = 0; i < CH; ++i) {
+            eh[i] = tab[1 * ts + q[i]];
+            a[i] = tab[2 * ts + q[i]];
+#pragma unroll
+            for (int f = 0; f < 3; ++f) uu[i][f] = u[f * fs + idx[i]];
+        }
+        if (st) {
+#pragma unroll
+            for (int i = 0; i < CH; ++i)
+#pragma unroll
+                for (int f = 0; f < 3; ++f) {
+                    const cpx<TS> x = nn[i][f];
+                    (STG == ST_1 ? n1 : sacc)[f * fs + idx[i]] = x;
+                    out[f * fs + idx[i]] = {TS(eh[i] * uu[i][f].re + dt * a[i] * x.re),
+                                            TS(eh[i] * uu[i][f].im + dt * a[i] * x.im)};
+                }
+        }
+    } else if constexpr (STG == ST_3) {
+        const double dt = sa.sc->dt;
+        double e[CH], a[CH];
+        cpx<TS> uu[CH][3], s2[CH][3], m1[CH][3];
+#pragma unroll
+        for (int i = 0; i < CH; ++i) {
+            e[i] = tab[0 * ts + q[i]];
+            a[i] = tab[3 * ts + q[i]];
+#pragma unroll
+            for (int f = 0; f < 3; ++f) {
+                uu[i][f] = u[f * fs + idx[i]];
+                s2[i][f] = sacc[f * fs + idx[i]];
+                m1[i][f] = n1[f * fs + idx[i]];
+            }
+        }
+        if (st) {
+#pragma unroll
+            for (int i = 0; i < CH; ++i)
+#pragma unroll
+                for (int f = 0; f < 3; ++f) {
+                    const cpx<TS> x = nn[i][f];
+                    sacc[f * fs + idx[i]] = {s2[i][f].re + x.re, s2[i][f].im + x.im};
+                    const double dr = 2.0 * x.re - m1[i][f].re, di = 2.0 * x.im - m1[i][f].im;
+                    out[f * fs + idx[i]] = {TS(e[i] * uu[i][f].re + dt * a[i] * dr),
+                                            TS(e[i] * uu[i][f].im + dt * a[i] * di)};
+                }
         }
     } else {  // ST_4: final combine into u
-        const double e = tab[0 * ts + q], b1 = tab[4 * ts + q], b23 = tab[5 * ts + q], b4 = tab[6 * ts + q];
-        for (int f = 0; f < 3; ++f) {
-            const cpx<TS> uu = u[f * sa.fs + idx];
-            const cpx<TS> m1 = n1[f * sa.fs + idx];
-            const cpx<TS> s = sacc[f * sa.fs + idx];
-            const cpx<TS> x = nn[f];
-            const double rr = b1 * m1.re + b23 * s.re + b4 * x.re;
-            const double ri = b1 * m1.im + b23 * s.im + b4 * x.im;
-            out[f * sa.fs + idx] = {TS(e * uu.re + dt * rr), TS(e * uu.im + dt * ri)};
+        static_assert(STG == ST_4, "unknown stage");
+        const double dt = sa.sc->dt;
+        double e[CH], b1[CH], b23[CH], b4[CH];
+        cpx<TS> uu[CH][3], m1[CH][3], s[CH][3];
+#pragma unroll
+        for (int i = 0; i < CH; ++i) {
+            e[i] = tab[0 * ts + q[i]];
+            b1[i] = tab[4 * ts + q[i]];
+            b23[i] = tab[5 * ts + q[i]];
+            b4[i] = tab[6 * ts + q[i]];
+#pragma unroll
+            for (int f = 0; f < 3; ++f) {
+                uu[i][f] = u[f * fs + idx[i]];
+                m1[i][f] = n1[f * fs + idx[i]];
+                s[i][f] = sacc[f * fs + idx[i]];
+            }
+        }
+        if (st) {
+#pragma unroll
+            for (int i = 0; i < CH; ++i)
+#pragma unroll
+                for (int f = 0; f < 3; ++f) {
+                    const cpx<TS> x = nn[i][f];
+                    const double rr = b1[i] * m1[i][f].re + b23[i] * s[i][f].re + b4[i] * x.re;
+                    const double ri = b1[i] * m1[i][f].im + b23[i] * s[i][f].im + b4[i] * x.im;
+                    out[f * fs + idx[i]] = {TS(e[i] * uu[i][f].re + dt * rr), TS(e[i] * uu[i][f].im + dt * ri)};
+                }
         }
     }
 }
@@ -245,8 +368,9 @@ __device__ __forceinline__ void leray_mode(double kx, double ky, double kz, cpx<
 //   AS_1 (stage u):  out = gu + t
 // where cot <- x stores inv_scale * P(x), the input the next VJP's inverse
 // passes expect.  All coefficient operators are real diagonal, hence self-adjoint.
-__device__ __forceinline__ void adj_epilogue(const StageArgs &sa, long long idx, int q, double kx, double ky,
-                                             double kz, const cpx<double> t[3]) {
+template <int STG, int CH>
+__device__ __forceinline__ void adj_chunk(const StageArgs &sa, bool st, const long long (&idx)[CH], const int (&q)[CH],
+                                          const double (&kv)[CH][3], const cpx<double> (&t)[CH][3]) {
     const long long fs = sa.fs;
     const cpx<double> *lam = reinterpret_cast<const cpx<double> *>(sa.lam);
     cpx<double> *gu = reinterpret_cast<cpx<double> *>(sa.gu);
@@ -255,46 +379,110 @@ __device__ __forceinline__ void adj_epilogue(const StageArgs &sa, long long idx,
     cpx<double> *out = reinterpret_cast<cpx<double> *>(sa.out);
     const double *tab = sa.tab;
     const int ts = sa.tab_stride;
-    const double dt = sa.sc->dt;
-    cpx<double> c[3];
-    if (sa.stage == AS_RAW) {
-        for (int f = 0; f < 3; ++f) out[f * fs + idx] = t[f];
-        return;
-    }
-    if (sa.stage == AS_1) {
-        for (int f = 0; f < 3; ++f) {
-            const cpx<double> g = gu[f * fs + idx];
-            out[f * fs + idx] = {g.re + t[f].re, g.im + t[f].im};
+    if constexpr (STG == AS_RAW) {
+        if (st) {
+#pragma unroll
+            for (int i = 0; i < CH; ++i)
+#pragma unroll
+                for (int f = 0; f < 3; ++f) out[f * fs + idx[i]] = t[i][f];
         }
         return;
+    } else if constexpr (STG == AS_1) {
+        cpx<double> g[CH][3];
+#pragma unroll
+        for (int i = 0; i < CH; ++i)
+#pragma unroll
+            for (int f = 0; f < 3; ++f) g[i][f] = gu[f * fs + idx[i]];
+        if (st) {
+#pragma unroll
+            for (int i = 0; i < CH; ++i)
+#pragma unroll
+                for (int f = 0; f < 3; ++f)
+                    out[f * fs + idx[i]] = {g[i][f].re + t[i][f].re, g[i][f].im + t[i][f].im};
+        }
+        return;
+    } else {
+        const double dt = sa.sc->dt;
+        cpx<double> c[CH][3];
+        if constexpr (STG == AS_4) {
+            double e[CH], a41[CH], b1[CH], b23[CH];
+            cpx<double> l[CH][3];
+#pragma unroll
+            for (int i = 0; i < CH; ++i) {
+                e[i] = tab[0 * ts + q[i]];
+                a41[i] = tab[3 * ts + q[i]];
+                b1[i] = tab[4 * ts + q[i]];
+                b23[i] = tab[5 * ts + q[i]];
+#pragma unroll
+                for (int f = 0; f < 3; ++f) l[i][f] = lam[f * fs + idx[i]];
+            }
+#pragma unroll
+            for (int i = 0; i < CH; ++i)
+#pragma unroll
+                for (int f = 0; f < 3; ++f) {
+                    const cpx<double> tt = t[i][f], ll = l[i][f];
+                    if (st) {
+                        gu[f * fs + idx[i]] = {e[i] * (ll.re + tt.re), e[i] * (ll.im + tt.im)};
+                        g1p[f * fs + idx[i]] = {dt * (b1[i] * ll.re - a41[i] * tt.re),
+                                                dt * (b1[i] * ll.im - a41[i] * tt.im)};
+                    }
+                    c[i][f] = {dt * (b23[i] * ll.re + 2.0 * a41[i] * tt.re), dt * (b23[i] * ll.im + 2.0 * a41[i] * tt.im)};
+                }
+        } else if constexpr (STG == AS_3) {
+            double eh[CH], a31[CH], b23[CH];
+            cpx<double> l[CH][3], g[CH][3];
+#pragma unroll
+            for (int i = 0; i < CH; ++i) {
+                eh[i] = tab[1 * ts + q[i]];
+                a31[i] = tab[2 * ts + q[i]];
+                b23[i] = tab[5 * ts + q[i]];
+#pragma unroll
+                for (int f = 0; f < 3; ++f) {
+                    l[i][f] = lam[f * fs + idx[i]];
+                    g[i][f] = gu[f * fs + idx[i]];
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < CH; ++i)
+#pragma unroll
+                for (int f = 0; f < 3; ++f) {
+                    const cpx<double> tt = t[i][f];
+                    if (st) gu[f * fs + idx[i]] = {g[i][f].re + eh[i] * tt.re, g[i][f].im + eh[i] * tt.im};
+                    c[i][f] = {dt * (b23[i] * l[i][f].re + a31[i] * tt.re), dt * (b23[i] * l[i][f].im + a31[i] * tt.im)};
+                }
+        } else {
+            static_assert(STG == AS_2, "unknown reverse stage");
+            double eh[CH], a21[CH];
+            cpx<double> g[CH][3], p[CH][3];
+#pragma unroll
+            for (int i = 0; i < CH; ++i) {
+                eh[i] = tab[1 * ts + q[i]];
+                a21[i] = tab[2 * ts + q[i]];
+#pragma unroll
+                for (int f = 0; f < 3; ++f) {
+                    g[i][f] = gu[f * fs + idx[i]];
+                    p[i][f] = g1p[f * fs + idx[i]];
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < CH; ++i)
+#pragma unroll
+                for (int f = 0; f < 3; ++f) {
+                    const cpx<double> tt = t[i][f];
+                    if (st) gu[f * fs + idx[i]] = {g[i][f].re + eh[i] * tt.re, g[i][f].im + eh[i] * tt.im};
+                    c[i][f] = {p[i][f].re + dt * a21[i] * tt.re, p[i][f].im + dt * a21[i] * tt.im};
+                }
+        }
+        if (st) {
+#pragma unroll
+            for (int i = 0; i < CH; ++i) {
+                leray_mode(kv[i][0], kv[i][1], kv[i][2], c[i]);
+#pragma unroll
+                for (int f = 0; f < 3; ++f)
+                    cot[f * fs + idx[i]] = {c[i][f].re * sa.inv_scale, c[i][f].im * sa.inv_scale};
+            }
+        }
     }
-    if (sa.stage == AS_4) {
-        const double e = tab[0 * ts + q], a41 = tab[3 * ts + q], b1 = tab[4 * ts + q], b23 = tab[5 * ts + q];
-        for (int f = 0; f < 3; ++f) {
-            const cpx<double> l = lam[f * fs + idx];
-            gu[f * fs + idx] = {e * (l.re + t[f].re), e * (l.im + t[f].im)};
-            g1p[f * fs + idx] = {dt * (b1 * l.re - a41 * t[f].re), dt * (b1 * l.im - a41 * t[f].im)};
-            c[f] = {dt * (b23 * l.re + 2.0 * a41 * t[f].re), dt * (b23 * l.im + 2.0 * a41 * t[f].im)};
-        }
-    } else if (sa.stage == AS_3) {
-        const double eh = tab[1 * ts + q], a31 = tab[2 * ts + q], b23 = tab[5 * ts + q];
-        for (int f = 0; f < 3; ++f) {
-            const cpx<double> l = lam[f * fs + idx];
-            const cpx<double> g = gu[f * fs + idx];
-            gu[f * fs + idx] = {g.re + eh * t[f].re, g.im + eh * t[f].im};
-            c[f] = {dt * (b23 * l.re + a31 * t[f].re), dt * (b23 * l.im + a31 * t[f].im)};
-        }
-    } else {  // AS_2
-        const double eh = tab[1 * ts + q], a21 = tab[2 * ts + q];
-        for (int f = 0; f < 3; ++f) {
-            const cpx<double> g = gu[f * fs + idx];
-            const cpx<double> p = g1p[f * fs + idx];
-            gu[f * fs + idx] = {g.re + eh * t[f].re, g.im + eh * t[f].im};
-            c[f] = {p.re + dt * a21 * t[f].re, p.im + dt * a21 * t[f].im};
-        }
-    }
-    leray_mode(kx, ky, kz, c);
-    for (int f = 0; f < 3; ++f) cot[f * fs + idx] = {c[f].re * sa.inv_scale, c[f].im * sa.inv_scale};
 }
 
 template <int L, int MODE>
@@ -310,10 +498,10 @@ struct FwdPlan<L, FWD_VJP> {
     using type = PlanE12<L>;
 };
 template <int L, int MODE, typename TW>
-using FwdGeom = Geom<L, FwdPlan<L, MODE>::type::E, TW>;
+using FwdGeom = Geom<L, FwdPlan<L, MODE>::type::E, TW, VK_ROW_BYTES_FWD>;
 
-template <int L, int LOUT, int MODE, int NF, typename TW>
-__global__ __launch_bounds__((FwdGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_waves_per_eu(Waves<K_FWD, TW>::value))) void k_fwd(const cpx<TW> *__restrict__ in,
+template <int L, int LOUT, int MODE, int NF, int STG, typename TW>
+__global__ __launch_bounds__((FwdGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_waves_per_eu(Waves<MODE == FWD_VJP ? K_FVJP : K_FWD, TW>::value))) void k_fwd(const cpx<TW> *__restrict__ in,
                                                                     cpx<TW> *__restrict__ out,
                                                                     const cpx<TW> *__restrict__ tw, Strided g,
                                                                     int n, double ks, TW scale, StageArgs sa) {
@@ -322,39 +510,56 @@ __global__ __launch_bounds__((FwdGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
     constexpr int E = P::E;
     constexpr int T = FwdGeom<L, MODE, TW>::T, NT = FwdGeom<L, MODE, TW>::NT;
     constexpr int RP = PL::last;
+    // the stage pass loads its next field while transforming the current one
+    // (twiddles from LDS, so no vector memory wait inside the transform holds
+    // for the prefetch)
+    constexpr bool PF = MODE == FWD_STAGE;
     __shared__ cpx<TW> lds[L * T];
+    __shared__ cpx<TW> ltw[PF ? L : 1];
     const int tid = threadIdx.x;
     const int col0 = blockIdx.x * T;
-    const cpx<TW> zero = {TW(0), TW(0)};
     cpx<TW> v[E];
+    if constexpr (PF) stage_twiddles<L, NT>(ltw, tw, tid);
     // kept outputs of the last radix under truncation: m in [0, RP/3) u [2RP/3, RP)
     constexpr bool KEEP = MODE == FWD_STAGE || MODE == FWD_VJP;
     constexpr int KR = KEEP ? 2 * RP / 3 : 1;
     constexpr int K = KEEP ? (E / RP) * KR : 1;
     static_assert(!KEEP || (RP % 3 == 0 && 3 * LOUT == 2 * L), "stage pass needs a 3/2-padded plan");
+    // kept modes per epilogue chunk (their loads are in flight together),
+    // sized so the chunk's loads fit beside the kept outputs r without
+    // spilling: stages 3 and 4 read three state arrays, the reverse stages
+    // read FP64 cotangents in both precisions
+    constexpr bool HEAVY = STG == ST_3 || STG == ST_4;
+    constexpr int CH = MODE == FWD_VJP ? 1 : sizeof(TW) == 8 ? (HEAVY ? 1 : 2) : (HEAVY ? 2 : 4);
+    static_assert(!KEEP || K % CH == 0, "chunks must tile the kept modes");
     cpx<TW> r[KEEP ? 3 : 1][K];
 
     const int col = col0 + tid % T;
     const bool colok = col < g.ncols && (col % g.hp) < g.hvalid;
     const long long cin = colok ? col_addr(col, g.in_inner, g.in_os) : 0;
     const long long cout = colok ? col_addr(col, g.out_inner, g.out_os) : 0;
-    auto load_field = [&](int f) {
+    auto load_into = [&](cpx<TW>(&S)[E], int f) {
         const gptr<cpx<TW>> src = opaque(in) + f * g.in_fs + cin;
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const int p = ci_pos_in<L, PL::first, T, NT>(tid, e);
-            v[e] = colok ? ldg(src, (long long)p * g.in_ls) : zero;
+            S[e] = ldg(src, (long long)p * g.in_ls);
         }
     };
-    // wavenumbers of kept element (b, mk)
-    auto kept_k = [&](int b, int mk, int &j, int &wx, int &wy, int &iz) {
+    auto load_field = [&](int f) { load_into(v, f); };
+    // kept element i = b * KR + mk: state-layout index, k^2 (table index; 0 for
+    // columns outside the geometry) and integer wavenumbers
+    auto kept = [&](int i, long long &idx, int &q, int &wx, int &wy, int &iz) {
+        const int b = i / KR, mk = i % KR;
         const int m = mk < RP / 3 ? mk : mk + RP / 3;
         const int p = ci_pos_out<L, RP, T, NT>(tid, b * RP + m);
-        j = unpad_index(p, LOUT, L);
+        const int j = unpad_index(p, LOUT, L);
         const int iy = col / g.hp;
         iz = col % g.hp;
         wx = wavenum(j, n);
         wy = wavenum(iy, n);
+        idx = cout + (long long)j * g.out_ls;
+        q = colok ? wx * wx + wy * wy + iz * iz : 0;
     };
     if constexpr (MODE == FWD_VJP) {
         // t = GU + curl(GW): GW (fields 3..5) first, curled in registers, then GU added
@@ -372,19 +577,17 @@ __global__ __launch_bounds__((FwdGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
             __syncthreads();
         }
 #pragma unroll
-        for (int b = 0; b < E / RP; ++b)
-#pragma unroll
-            for (int mk = 0; mk < KR; ++mk) {
-                int j, wx, wy, iz;
-                kept_k(b, mk, j, wx, wy, iz);
-                const TW kx = TW(double(wx) * ks), ky = TW(double(wy) * ks), kz = TW(double(iz) * ks);
-                const int i = b * KR + mk;
-                const cpx<TW> w0 = r[0][i], w1 = r[1][i], w2 = r[2][i];
-                // SpectralOps::curl (vonkarman-core/src/spectral_ops.rs:93-104)
-                r[0][i] = {-(ky * w2.im - kz * w1.im), ky * w2.re - kz * w1.re};
-                r[1][i] = {-(kz * w0.im - kx * w2.im), kz * w0.re - kx * w2.re};
-                r[2][i] = {-(kx * w1.im - ky * w0.im), kx * w1.re - ky * w0.re};
-            }
+        for (int i = 0; i < K; ++i) {
+            long long idx;
+            int q, wx, wy, iz;
+            kept(i, idx, q, wx, wy, iz);
+            const TW kx = TW(double(wx) * ks), ky = TW(double(wy) * ks), kz = TW(double(iz) * ks);
+            const cpx<TW> w0 = r[0][i], w1 = r[1][i], w2 = r[2][i];
+            // SpectralOps::curl (vonkarman-core/src/spectral_ops.rs:93-104)
+            r[0][i] = {-(ky * w2.im - kz * w1.im), ky * w2.re - kz * w1.re};
+            r[1][i] = {-(kz * w0.im - kx * w2.im), kz * w0.re - kx * w2.re};
+            r[2][i] = {-(kx * w1.im - ky * w0.im), kx * w1.re - ky * w0.re};
+        }
 #pragma unroll
         for (int f = 0; f < 3; ++f) {
             load_field(f);
@@ -398,25 +601,32 @@ __global__ __launch_bounds__((FwdGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
                 }
             __syncthreads();
         }
-        if (colok) {
 #pragma unroll
-            for (int b = 0; b < E / RP; ++b)
+        for (int c0 = 0; c0 < K; c0 += CH) {
+            long long idx[CH];
+            int q[CH];
+            double kv[CH][3];
+            cpx<double> tt[CH][3];
 #pragma unroll
-                for (int mk = 0; mk < KR; ++mk) {
-                    int j, wx, wy, iz;
-                    kept_k(b, mk, j, wx, wy, iz);
-                    const int i = b * KR + mk;
-                    const cpx<double> tt[3] = {cvt<double>(r[0][i]), cvt<double>(r[1][i]), cvt<double>(r[2][i])};
-                    adj_epilogue(sa, cout + (long long)j * g.out_ls, wx * wx + wy * wy + iz * iz,
-                                 double(wx) * ks, double(wy) * ks, double(iz) * ks, tt);
-                }
+            for (int i = 0; i < CH; ++i) {
+                int wx, wy, iz;
+                kept(c0 + i, idx[i], q[i], wx, wy, iz);
+                kv[i][0] = double(wx) * ks;
+                kv[i][1] = double(wy) * ks;
+                kv[i][2] = double(iz) * ks;
+#pragma unroll
+                for (int f = 0; f < 3; ++f) tt[i][f] = cvt<double>(r[f][c0 + i]);
+            }
+            adj_chunk<STG, CH>(sa, colok, idx, q, kv, tt);
         }
     } else if constexpr (MODE == FWD_STAGE) {
         // three fields; the kept outputs of each stay in registers (static indices)
+        cpx<TW> Q[E];
+        load_field(0);
 #pragma unroll
         for (int f = 0; f < 3; ++f) {
-            load_field(f);
-            fft_reg<L, -1, T, NT, T, 1, PL>(v, lds, tw, tid);
+            if (f + 1 < 3) load_into(Q, f + 1);
+            fft_reg_lt<L, -1, T, NT, T, 1, PL>(v, lds, ltw, tid);
 #pragma unroll
             for (int b = 0; b < E / RP; ++b)
 #pragma unroll
@@ -425,6 +635,35 @@ __global__ __launch_bounds__((FwdGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
                     r[f][b * KR + mk] = cscale(v[b * RP + m], scale);
                 }
             __syncthreads();
+#pragma unroll
+            for (int e = 0; e < E; ++e) v[e] = Q[e];
+        }
+#pragma unroll
+        for (int c0 = 0; c0 < K; c0 += CH) {
+            long long idx[CH];
+            int q[CH];
+            cpx<TW> nn[CH][3];
+#pragma unroll
+            for (int i = 0; i < CH; ++i) {
+                int wx, wy, iz;
+                kept(c0 + i, idx[i], q[i], wx, wy, iz);
+                const TW kx = TW(double(wx) * ks), ky = TW(double(wy) * ks), kz = TW(double(iz) * ks);
+                // Leray projection (vonkarman-core/src/spectral_ops.rs:114-151)
+                cpx<TW> x = r[0][c0 + i], y = r[1][c0 + i], z = r[2][c0 + i];
+                const TW k2 = kx * kx + ky * ky + kz * kz;
+                if (!(double(k2) < 1e-30)) {
+                    const TW inv = TW(1) / k2;
+                    const TW dr = kx * x.re + ky * y.re + kz * z.re;
+                    const TW di = kx * x.im + ky * y.im + kz * z.im;
+                    x = {x.re - kx * dr * inv, x.im - kx * di * inv};
+                    y = {y.re - ky * dr * inv, y.im - ky * di * inv};
+                    z = {z.re - kz * dr * inv, z.im - kz * di * inv};
+                }
+                nn[i][0] = x;
+                nn[i][1] = y;
+                nn[i][2] = z;
+            }
+            stage_chunk<STG, CH, TW>(sa, colok, idx, q, nn);
         }
     } else {
         for (int f = 0; f < NF; ++f) {
@@ -441,34 +680,6 @@ __global__ __launch_bounds__((FwdGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
             __syncthreads();
         }
     }
-    if constexpr (MODE == FWD_STAGE) {
-#pragma unroll
-        for (int b = 0; b < E / RP; ++b)
-#pragma unroll
-            for (int mk = 0; mk < KR; ++mk) {
-                const int m = mk < RP / 3 ? mk : mk + RP / 3;
-                if (!colok) continue;
-                const int p = ci_pos_out<L, RP, T, NT>(tid, b * RP + m);
-                const int j = unpad_index(p, LOUT, L);
-                const int iy = col / g.hp, iz = col % g.hp;
-                const int wx = wavenum(j, n), wy = wavenum(iy, n);
-                const TW kx = TW(double(wx) * ks), ky = TW(double(wy) * ks), kz = TW(double(iz) * ks);
-                // Leray projection (vonkarman-core/src/spectral_ops.rs:114-151)
-                cpx<TW> x = r[0][b * KR + mk], y = r[1][b * KR + mk], z = r[2][b * KR + mk];
-                const TW k2 = kx * kx + ky * ky + kz * kz;
-                if (!(double(k2) < 1e-30)) {
-                    const TW inv = TW(1) / k2;
-                    const TW dr = kx * x.re + ky * y.re + kz * z.re;
-                    const TW di = kx * x.im + ky * y.im + kz * z.im;
-                    x = {x.re - kx * dr * inv, x.im - kx * di * inv};
-                    y = {y.re - ky * dr * inv, y.im - ky * di * inv};
-                    z = {z.re - kz * dr * inv, z.im - kz * di * inv};
-                }
-                const cpx<TW> nn[3] = {x, y, z};
-                const long long idx = cout + (long long)j * g.out_ls;
-                stage_epilogue<TW>(sa, idx, wx * wx + wy * wy + iz * iz, nn);
-            }
-    }
 }
 
 // ===========================================================================
@@ -480,33 +691,70 @@ __global__ __launch_bounds__((FwdGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
 // (line-major tile, pitch L); registers are filled straight from HBM.
 // ===========================================================================
 // Hermitian-extended pair spectrum Z at FFT position p from the half
-// spectra fa, fb (HIN stored entries; fb may be null)
-template <int L, int HIN, typename TW>
-__device__ __forceinline__ cpx<TW> z_pair_at(gptr<cpx<TW>> fa, gptr<cpx<TW>> fb, long long base, int p) {
-    if (p == 0 || 2 * p == L) {
-        if (p >= HIN) return {TW(0), TW(0)};
-        return {fa[base + p].re, fb ? fb[base + p].re : TW(0)};
-    }
-    if (p < HIN) {
+// spectra fa, fb (HIN stored entries; fb absent when HASB is false):
+//   p < HIN (stored):        Z = X_p + i Y_p      (Im X, Im Y dropped at p = 0, L/2)
+//   p > L - HIN (mirrored):  Z = conj X_{L-p} + i conj Y_{L-p}
+//   otherwise (zero gap):    Z = 0
+// CLS classifies the whole position range of the load at compile time
+// (zpair_class); in the mixed class the loads are unconditional from a
+// clamped index and the cases are selected afterwards, because a load under a
+// branch is waited for before the branch joins.
+enum { ZP_DIRECT = 0, ZP_MIRROR = 1, ZP_ZERO = 2, ZP_MIXED = 3 };
+template <int L, int HIN>
+constexpr int zpair_class(int b, int w) {
+    const int e = b + w;  // exclusive end of the range
+    const bool edge = b == 0 || (b <= L / 2 && L / 2 < e);
+    if (!edge && b >= 1 && e <= HIN) return ZP_DIRECT;
+    if (!edge && b >= HIN && b > L - HIN && e <= L) return ZP_MIRROR;
+    if (b >= HIN && e - 1 <= L - HIN) return ZP_ZERO;
+    return ZP_MIXED;
+}
+template <int L, int HIN, bool HASB, int CLS, typename TW>
+__device__ __forceinline__ cpx<TW> z_pair_ld(gptr<cpx<TW>> fa, gptr<cpx<TW>> fb, long long base, int p) {
+    const cpx<TW> zero = {TW(0), TW(0)};
+    if constexpr (CLS == ZP_ZERO) {
+        return zero;
+    } else if constexpr (CLS == ZP_DIRECT) {
         const cpx<TW> X = ldg(fa, base + p);
-        const cpx<TW> Y = fb ? ldg(fb, base + p) : cpx<TW>{TW(0), TW(0)};
+        const cpx<TW> Y = HASB ? ldg(fb, base + p) : zero;
         return {X.re - Y.im, X.im + Y.re};
-    }
-    if (p > L - HIN) {
-        const int k = L - p;
-        const cpx<TW> X = ldg(fa, base + k);
-        const cpx<TW> Y = fb ? ldg(fb, base + k) : cpx<TW>{TW(0), TW(0)};
+    } else if constexpr (CLS == ZP_MIRROR) {
+        const cpx<TW> X = ldg(fa, base + (L - p));
+        const cpx<TW> Y = HASB ? ldg(fb, base + (L - p)) : zero;
         return {X.re + Y.im, Y.re - X.im};
+    } else {
+        const bool lo = p < HIN, hi = !lo && p > L - HIN, edge = p == 0 || 2 * p == L;
+        const int k = lo ? p : (hi ? L - p : 0);
+        const cpx<TW> X = ldg(fa, base + k);
+        const cpx<TW> Y = HASB ? ldg(fb, base + k) : zero;
+        const TW xi = edge ? TW(0) : X.im, yi = edge ? TW(0) : Y.im;
+        const TW zr = lo ? X.re - yi : (hi ? X.re + yi : TW(0));
+        const TW zi = lo ? xi + Y.re : (hi ? Y.re - xi : TW(0));
+        return {zr, zi};
     }
-    return {TW(0), TW(0)};
+}
+// fill the registers of a column-interleaved z tile (T lines, NT threads,
+// first radix R1) with the pair spectrum of line `base` (element e at
+// position ci_pos_in(tid, e))
+template <int L, int HIN, int R1, int T, int NT, bool HASB, int E, typename TW>
+__device__ __forceinline__ void z_pair_fill(cpx<TW> (&v)[E], gptr<cpx<TW>> fa, gptr<cpx<TW>> fb, long long base,
+                                            int tid) {
+    static_for<E>([&](auto ec) {
+        constexpr int e = decltype(ec)::value;
+        constexpr int cls = zpair_class<L, HIN>(ci_in_base<L, R1, T, NT>(e), NT / T);
+        v[e] = z_pair_ld<L, HIN, HASB, cls, TW>(fa, fb, base, ci_pos_in<L, R1, T, NT>(tid, e));
+    });
 }
 
 // z tiles are column-interleaved too: the T lines of a block are the columns,
-// so position p of all T lines is one contiguous LDS row.  64-byte rows (small
-// blocks) let more blocks share a CU; each line still reads HBM in runs of
-// 16 consecutive positions across a wavefront.
+// so position p of all T lines is one contiguous LDS row.  32-byte rows make
+// a block of one wavefront (T = 2 FP64 lines, NT = 64): its barriers cost
+// almost nothing and blocks on a SIMD run independently.  Each line still
+// reads HBM in runs of 32 consecutive positions per load instruction.
+// Measured at 384 (256^3 solver), per launch: FP64 1.03 ms with 64-byte rows
+// -> 0.87 ms with 32-byte rows; FP32 0.71 -> 0.65 ms.
 #ifndef VK_Z_ROW_BYTES
-#define VK_Z_ROW_BYTES 64  // T = 4 (FP64) or 8 (FP32) lines per block
+#define VK_Z_ROW_BYTES 32  // T = 2 (FP64) or 4 (FP32) lines per block
 #endif
 template <int L, typename TW>
 using ZNLP = PlanE12<L>;
@@ -529,7 +777,8 @@ __global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) __attribute__((amdgpu_waves_p
     const long long line0 = (long long)blockIdx.x * T;
     const int t = tid % T;
     const bool ok = line0 + t < lines;
-    const long long base = (line0 + t) * hp;
+    // lines past the end load the last line (valid memory); their results are not stored
+    const long long base = (ok ? line0 + t : lines - 1) * hp;
     cpx<TW> v[E];
     TW r[6][E];
 
@@ -537,11 +786,7 @@ __global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) __attribute__((amdgpu_waves_p
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
         const gptr<cpx<TW>> fa = opaque(in) + (2 * q) * fs_in, fb = opaque(in) + (2 * q + 1) * fs_in;
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const int p = ci_pos_in<L, PL::first, T, NT>(tid, e);
-            v[e] = ok ? z_pair_at<L, HIN, TW>(fa, fb, base, p) : cpx<TW>{TW(0), TW(0)};
-        }
+        z_pair_fill<L, HIN, PL::first, T, NT, true>(v, fa, fb, base, tid);
         fft_reg<L, +1, T, NT, T, 1, PL>(v, lds, tw, tid);
 #pragma unroll
         for (int e = 0; e < E; ++e) {
@@ -592,14 +837,6 @@ __global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) __attribute__((amdgpu_waves_p
     }
 }
 
-// Pair spectrum with per-half DC factors (the z transpose of an R2C counts
-// its k=0 term once where the C2R extension would count it twice).
-template <int L, int HIN, typename TW>
-__device__ __forceinline__ cpx<TW> z_pair_dc(gptr<cpx<TW>> fa, gptr<cpx<TW>> fb, long long base, int p, TW dca,
-                                            TW dcb) {
-    const cpx<TW> z = z_pair_at<L, HIN, TW>(fa, fb, base, p);
-    return p == 0 ? cpx<TW>{z.re * dca, z.im * dcb} : z;
-}
 
 // split the forward DFT of (a + i b) held in registers into the truncated
 // half spectra of a and b, times c_k s (c_0 = 1, c_k = 2 otherwise): the
@@ -643,18 +880,21 @@ __global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) __attribute__((amdgpu_waves_p
     const long long line0 = (long long)blockIdx.x * T;
     const int t = tid % T;
     const bool ok = line0 + t < lines;
-    const long long base = (line0 + t) * hp;
+    // lines past the end load the last line (valid memory); their results are not stored
+    const long long base = (ok ? line0 + t : lines - 1) * hp;
     const TW one = TW(1), two = TW(2), half = TW(0.5);
     cpx<TW> v[E];
     TW g0[E], g1[E], r0[E], r1[E], r2[E], r3[E];
 
+    // inverse pair transform of fields (fa, fb) (fb < 0: none) with per-half
+    // DC factors: the z transpose of an R2C counts its k=0 term once where the
+    // C2R extension would count it twice
     auto inverse = [&](int fa, int fb, TW dca, TW dcb) {
-        const gptr<cpx<TW>> pa = opaque(in) + fa * fs_in, pb = fb >= 0 ? opaque(in) + fb * fs_in : nullptr;
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const int p = ci_pos_in<L, PL::first, T, NT>(tid, e);
-            v[e] = ok ? z_pair_dc<L, HIN, TW>(pa, pb, base, p, dca, dcb) : cpx<TW>{TW(0), TW(0)};
-        }
+        const gptr<cpx<TW>> pa = opaque(in) + fa * fs_in;
+        if (fb >= 0) z_pair_fill<L, HIN, PL::first, T, NT, true>(v, pa, opaque(in) + fb * fs_in, base, tid);
+        else z_pair_fill<L, HIN, PL::first, T, NT, false>(v, pa, pa, base, tid);
+        // position 0 is element 0 of thread column 0's first row
+        if (ci_pos_in<L, PL::first, T, NT>(tid, 0) == 0) v[0] = {v[0].re * dca, v[0].im * dcb};
         fft_reg<L, +1, T, NT, T, 1, PL>(v, lds, tw, tid);
     };
     // Cotangent fields are paired only with cotangent fields: a complex FFT
@@ -734,17 +974,15 @@ __global__ __launch_bounds__((ZMaxGeom<L, TW>::NT)) void k_z_max(const cpx<TW> *
     const long long line0 = (long long)blockIdx.x * T;
     const int t = tid % T;
     const bool ok = line0 + t < lines;
-    const long long base = (line0 + t) * hp;
+    // lines past the end load the last line (valid memory) and are left out of the maximum
+    const long long base = (ok ? line0 + t : lines - 1) * hp;
     cpx<TW> v[E];
     TW r[3][E];
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
-        const gptr<cpx<TW>> fa = opaque(in) + (2 * q) * fs, fb = q == 0 ? opaque(in) + fs : nullptr;
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const int p = ci_pos_in<L, PL::first, T, NT>(tid, e);
-            v[e] = ok ? z_pair_at<L, HIN, TW>(fa, fb, base, p) : cpx<TW>{TW(0), TW(0)};
-        }
+        const gptr<cpx<TW>> fa = opaque(in) + (2 * q) * fs;
+        if (q == 0) z_pair_fill<L, HIN, PL::first, T, NT, true>(v, fa, opaque(in) + fs, base, tid);
+        else z_pair_fill<L, HIN, PL::first, T, NT, false>(v, fa, fa, base, tid);
         fft_reg<L, +1, T, NT, T, 1, PL>(v, lds, tw, tid);
 #pragma unroll
         for (int e = 0; e < E; ++e) {
@@ -788,17 +1026,15 @@ __global__ __launch_bounds__((ZSGeom<L, TW>::NT)) void k_z_c2r(const cpx<TW> *__
     const int tid = threadIdx.x;
     const long long s = (long long)blockIdx.x * T + tid % T;
     const long long la = 2 * s, lb = la + 1;
-    cpx<TW> v[E];
+    cpx<TW> v[E], vb[E];
+    // lines past the end load the last line (valid memory); b is zeroed past the end
+    const gptr<cpx<TW>> src = opaque(in);
+    z_pair_fill<L, HIN, PL::first, T, NT, false>(v, src, src, (la < lines ? la : lines - 1) * hp, tid);
+    z_pair_fill<L, HIN, PL::first, T, NT, false>(vb, src, src, (lb < lines ? lb : lines - 1) * hp, tid);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-        const int p = ci_pos_in<L, PL::first, T, NT>(tid, e);
-        v[e] = {TW(0), TW(0)};
-        if (la < lines) {
-            const gptr<cpx<TW>> src = opaque(in), none = nullptr;
-            const cpx<TW> za = z_pair_at<L, HIN, TW>(src, none, la * hp, p);
-            const cpx<TW> zb = lb < lines ? z_pair_at<L, HIN, TW>(src, none, lb * hp, p) : cpx<TW>{TW(0), TW(0)};
-            v[e] = {za.re - zb.im, za.im + zb.re};  // X_a + i X_b
-        }
+        const cpx<TW> zb = lb < lines ? vb[e] : cpx<TW>{TW(0), TW(0)};
+        v[e] = {v[e].re - zb.im, v[e].im + zb.re};  // X_a + i X_b
     }
     fft_reg<L, +1, T, NT, T, 1, PL>(v, lds, tw, tid);
 #pragma unroll
@@ -908,17 +1144,43 @@ int launch_inv_L(int mode, int lin, Prec ip, Prec wp, const void *in, void *out,
     return 2;
 }
 
-template <int L, int LOUT, int MODE, int NF, typename TW>
+template <int L, int LOUT, int MODE, int NF, int STG, typename TW>
 int run_fwd(const void *in, void *out, const void *tw, const Strided &g, int n, double ks, double scale,
             const StageArgs *sa, hipStream_t st) {
     using C = FwdGeom<L, MODE, TW>;
     const unsigned nb = (unsigned)((g.ncols + C::T - 1) / C::T);
     StageArgs a{};
     if (sa) a = *sa;
-    hipLaunchKernelGGL((k_fwd<L, LOUT, MODE, NF, TW>), dim3(nb), dim3(C::NT), 0, st,
+    hipLaunchKernelGGL((k_fwd<L, LOUT, MODE, NF, STG, TW>), dim3(nb), dim3(C::NT), 0, st,
                        reinterpret_cast<const cpx<TW> *>(in), reinterpret_cast<cpx<TW> *>(out),
                        reinterpret_cast<const cpx<TW> *>(tw), g, n, ks, TW(scale), a);
     return hipGetLastError() == hipSuccess ? 0 : 3;
+}
+
+// stage passes: the runtime stage selects the kernel instance
+template <int L, int LOUT, typename TW>
+int run_fwd_stage(const void *in, void *out, const void *tw, const Strided &g, int n, double ks, double scale,
+                  const StageArgs *sa, hipStream_t st) {
+    switch (sa->stage) {
+        case ST_RAW: return run_fwd<L, LOUT, FWD_STAGE, 3, ST_RAW, TW>(in, out, tw, g, n, ks, scale, sa, st);
+        case ST_1: return run_fwd<L, LOUT, FWD_STAGE, 3, ST_1, TW>(in, out, tw, g, n, ks, scale, sa, st);
+        case ST_2: return run_fwd<L, LOUT, FWD_STAGE, 3, ST_2, TW>(in, out, tw, g, n, ks, scale, sa, st);
+        case ST_3: return run_fwd<L, LOUT, FWD_STAGE, 3, ST_3, TW>(in, out, tw, g, n, ks, scale, sa, st);
+        case ST_4: return run_fwd<L, LOUT, FWD_STAGE, 3, ST_4, TW>(in, out, tw, g, n, ks, scale, sa, st);
+    }
+    return 2;
+}
+template <int L, int LOUT, typename TW>
+int run_fwd_vjp(const void *in, void *out, const void *tw, const Strided &g, int n, double ks, double scale,
+                const StageArgs *sa, hipStream_t st) {
+    switch (sa->stage) {
+        case AS_RAW: return run_fwd<L, LOUT, FWD_VJP, 6, AS_RAW, TW>(in, out, tw, g, n, ks, scale, sa, st);
+        case AS_1: return run_fwd<L, LOUT, FWD_VJP, 6, AS_1, TW>(in, out, tw, g, n, ks, scale, sa, st);
+        case AS_2: return run_fwd<L, LOUT, FWD_VJP, 6, AS_2, TW>(in, out, tw, g, n, ks, scale, sa, st);
+        case AS_3: return run_fwd<L, LOUT, FWD_VJP, 6, AS_3, TW>(in, out, tw, g, n, ks, scale, sa, st);
+        case AS_4: return run_fwd<L, LOUT, FWD_VJP, 6, AS_4, TW>(in, out, tw, g, n, ks, scale, sa, st);
+    }
+    return 2;
 }
 
 template <int L>
@@ -926,27 +1188,27 @@ int launch_fwd_L(int mode, int lout, int nf, Prec wp, const void *in, void *out,
                  const Strided &g, int n, double ks, double scale, const StageArgs *sa, hipStream_t st) {
     if (lout == L) {
         if (mode == FWD_PLAIN && nf == 1 && wp == F64)
-            return run_fwd<L, L, FWD_PLAIN, 1, double>(in, out, tw, g, n, ks, scale, sa, st);
+            return run_fwd<L, L, FWD_PLAIN, 1, 0, double>(in, out, tw, g, n, ks, scale, sa, st);
         return 2;
     }
     if constexpr (L % 3 == 0 && (2 * L / 3) % 2 == 0) {
         constexpr int LOUT = 2 * L / 3;
         if (lout != LOUT) return 2;
         if (mode == FWD_PLAIN && nf == 3) {
-            if (wp == F64) return run_fwd<L, LOUT, FWD_PLAIN, 3, double>(in, out, tw, g, n, ks, scale, sa, st);
-            return run_fwd<L, LOUT, FWD_PLAIN, 3, float>(in, out, tw, g, n, ks, scale, sa, st);
+            if (wp == F64) return run_fwd<L, LOUT, FWD_PLAIN, 3, 0, double>(in, out, tw, g, n, ks, scale, sa, st);
+            return run_fwd<L, LOUT, FWD_PLAIN, 3, 0, float>(in, out, tw, g, n, ks, scale, sa, st);
         }
         if (mode == FWD_STAGE && nf == 3 && sa) {
-            if (wp == F64) return run_fwd<L, LOUT, FWD_STAGE, 3, double>(in, out, tw, g, n, ks, scale, sa, st);
-            return run_fwd<L, LOUT, FWD_STAGE, 3, float>(in, out, tw, g, n, ks, scale, sa, st);
+            if (wp == F64) return run_fwd_stage<L, LOUT, double>(in, out, tw, g, n, ks, scale, sa, st);
+            return run_fwd_stage<L, LOUT, float>(in, out, tw, g, n, ks, scale, sa, st);
         }
         if (mode == FWD_PLAIN && nf == 6) {
-            if (wp == F64) return run_fwd<L, LOUT, FWD_PLAIN, 6, double>(in, out, tw, g, n, ks, scale, sa, st);
-            return run_fwd<L, LOUT, FWD_PLAIN, 6, float>(in, out, tw, g, n, ks, scale, sa, st);
+            if (wp == F64) return run_fwd<L, LOUT, FWD_PLAIN, 6, 0, double>(in, out, tw, g, n, ks, scale, sa, st);
+            return run_fwd<L, LOUT, FWD_PLAIN, 6, 0, float>(in, out, tw, g, n, ks, scale, sa, st);
         }
         if (mode == FWD_VJP && nf == 6 && sa) {
-            if (wp == F64) return run_fwd<L, LOUT, FWD_VJP, 6, double>(in, out, tw, g, n, ks, scale, sa, st);
-            return run_fwd<L, LOUT, FWD_VJP, 6, float>(in, out, tw, g, n, ks, scale, sa, st);
+            if (wp == F64) return run_fwd_vjp<L, LOUT, double>(in, out, tw, g, n, ks, scale, sa, st);
+            return run_fwd_vjp<L, LOUT, float>(in, out, tw, g, n, ks, scale, sa, st);
         }
     }
     return 2;
