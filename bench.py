@@ -29,25 +29,37 @@ HBM_PEAK_GBS = 8000.0
 
 def pass_bytes(n: int, w: int) -> dict:
     """Algorithmic HBM bytes per launch of each pass (valid spectral entries
-    only; complex = 2w bytes).  N = n, M = 3n/2, H = n/2+1."""
+    only; complex = 2w bytes, w = 8 FP64 or 4 mixed; the state and stage
+    arrays are in the work precision).  N = n, M = 3n/2, H = n/2+1.
+    One state-layout unit U = 3 N N H complex.  Stage epilogue traffic in U:
+    ST_1 reads u, writes N1 and the stage input; ST_2 reads u, writes N2 and
+    the stage input; ST_3 reads u, N1, N2 and writes N2+N3 and the stage input;
+    ST_4 reads u, N1, N2+N3 and writes u.  The fused x pass (stages 1-3) does
+    not write the stage input and writes the next evaluation's x-inverse
+    output (5 fields of M N H) instead of that evaluation's x pass reading it."""
     N, M, H = n, 3 * n // 2, n // 2 + 1
     c = 2 * w
-    state = 16  # the state and stage arrays stay FP64
     nnh, mnh, mmh = N * N * H, M * N * H, M * M * H
-    stage_avg = (3 + 3 + 5 + 4) / 4.0  # u/N1/N2+N3/stage-input arrays touched per stage
+    U = 3 * nnh * c
     return {
-        "x_inverse": 3 * nnh * state + 5 * mnh * c,
+        "x_inverse": U + 5 * mnh * c,
         "y_inverse": 5 * mnh * c + 6 * mmh * c,
         "z_cross": 6 * mmh * c + 3 * mmh * c,
         "y_forward": 3 * mmh * c + 3 * mnh * c,
-        "x_forward": 3 * mnh * c + stage_avg * 3 * nnh * state,
-        "cfl": 3 * nnh * state + 3 * nnh * c + 3 * nnh * c + 3 * nnh * c + 3 * nnh * c,
+        "x_forward": 3 * mnh * c + 4 * U,                                # stage 4
+        "x_forward_unfused": 3 * mnh * c + (3 + 3 + 5 + 4) / 4.0 * U,    # average of stages 1-4
+        "x_fused": 3 * mnh * c + (2 + 2 + 4) / 3.0 * U + 5 * mnh * c,    # average of stages 1-3
+        "cfl": U + 3 * nnh * c + 3 * nnh * c + 3 * nnh * c + 3 * nnh * c,
     }
 
 
-def step_bytes(n: int, w: int, adaptive: bool = True) -> float:
+def step_bytes(n: int, w: int, adaptive: bool = True, fused: bool = True) -> float:
     b = pass_bytes(n, w)
-    s = 4 * (b["x_inverse"] + b["y_inverse"] + b["z_cross"] + b["y_forward"] + b["x_forward"])
+    s = 4 * (b["y_inverse"] + b["z_cross"] + b["y_forward"])
+    if fused:
+        s += b["x_inverse"] + 3 * b["x_fused"] + b["x_forward"]
+    else:
+        s += 4 * (b["x_inverse"] + b["x_forward_unfused"])
     return s + (b["cfl"] if adaptive else 0.0)
 
 
@@ -55,6 +67,21 @@ def fft_c2r_bytes(m: int, w: int = 8) -> float:
     """One unfused 3D C2R at m^3: 5 spectral sweeps (2w bytes each) and the
     real output (w bytes per point)."""
     return 2 * w * 5 * m * m * (m // 2 + 1) + w * m ** 3
+
+
+def timing_dict(t) -> dict:
+    """Average device ms per launch of each pass from a vk_timing record (4
+    nonlinear evaluations per step; with the fused x pass, stages 1-3 run
+    x_fused instead of x_forward and the next evaluation's x_inverse)."""
+    nl = 4 * t.steps
+    avg = {"y_inverse": t.y_inverse / nl, "z_cross": t.z_cross / nl, "y_forward": t.y_forward / nl}
+    for k, n in (("x_inverse", t.n_x_inverse), ("x_forward", t.n_x_forward), ("x_fused", t.n_x_fused)):
+        if n:
+            avg[k] = getattr(t, k) / n
+    return {"steps": t.steps, "ms_per_step_instrumented": t.total / t.steps, "avg_ms": avg,
+            "launches_per_step": {"x_inverse": t.n_x_inverse / t.steps, "x_forward": t.n_x_forward / t.steps,
+                                  "x_fused": t.n_x_fused / t.steps},
+            "cfl_ms_per_step": t.cfl / t.steps}
 
 
 def run_gpu(args):
@@ -105,14 +132,7 @@ def run_gpu(args):
     timing = {}
     for prec in ("f64", "mixed"):
         t = out[prec]["solver"].steps_timed(max(2, min(10, args.steps)))
-        launches = 4 * t.steps
-        timing[prec] = {
-            "steps": t.steps,
-            "ms_per_step_instrumented": t.total / t.steps,
-            "avg_ms": {k: getattr(t, k) / launches for k in ("x_inverse", "y_inverse", "z_cross", "y_forward",
-                                                               "x_forward")},
-            "cfl_ms_per_step": t.cfl / t.steps,
-        }
+        timing[prec] = timing_dict(t)
 
     # standalone FFT at the padded size
     m = 3 * n // 2
@@ -414,8 +434,8 @@ KERNEL_OF_PASS = {  # kernel-name prefixes of the FP64 passes at the padded leng
     "y_inverse": "k_inv<{M}, {N}, 4, double, double>",
     "z_cross": "k_z_nl<{M}, double>",
     "y_forward": "k_fwd<{M}, {N}, 0, 3, 0, double>",
-    "x_forward": "k_fwd<{M}, {N}, 1, 3, ",  # the four stage instances
-
+    "x_forward": "k_fwd<{M}, {N}, 1, 3, ",  # the stage instances
+    "x_fused": "k_fwd_stage_inv<{M}, {N}, ",
 }
 
 
@@ -556,6 +576,7 @@ def main():
     adj = adjoint_cost(n) if not args.no_adjoint else None
     cpu = None if args.no_cpu else cpu_baseline(n)
     sb = step_bytes(n, 8)
+    pass_gbps = {k: pb[k] / (avg[k] * 1e-3) / 1e9 for k in avg}
     line = {
         "metric": f"ETD-RK4 steps/s @{n}^3 FP64 ({3 * n // 2}^3 padded)",
         "value": sps,
@@ -580,6 +601,7 @@ def main():
         "step_hbm_frac": sb * sps / (HBM_PEAK_GBS * 1e9),
         "step_bytes": sb,
         "passes_f64": timing["f64"],
+        "passes_f64_GBps": pass_gbps,
         "passes_mixed": timing["mixed"],
         "mixed": {"steps_per_s": sps_mixed, "speedup_vs_f64": sps_mixed / sps,
                   "energy_rel_err_vs_f64": acc["energy_rel_err"], "field_rel_l2_vs_f64": acc["field_rel_l2"],

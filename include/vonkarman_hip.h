@@ -104,6 +104,11 @@ typedef struct vk_timing {
     double cfl;         /* CFL max|u| passes, dt update, phi table  */
     double total;       /* whole timed region                       */
     int steps;
+    int n_x_inverse;    /* launches summed in x_inverse             */
+    int n_x_forward;    /* launches summed in x_forward             */
+    int n_x_fused;      /* launches summed in x_fused               */
+    double x_fused;     /* x forward + stage + the next stage's x
+                           inverse in one pass (stages 1-3)         */
 } vk_timing;
 
 /* Periodic3D::new without the IC (solver.rs:38-108): an n^3 grid on a box of
@@ -131,6 +136,13 @@ int vk_solver_set_clock(vk_solver *s, double time, uint64_t step);
 int vk_solver_step(vk_solver *s, int nsteps);
 /* vk_solver_step with per-pass device timing accumulated into *t. */
 int vk_solver_step_timed(vk_solver *s, int nsteps, vk_timing *t);
+/* No counterpart in the reference (a performance switch of this backend):
+ * enable (1, the default) or disable (0) the fused x pass that runs each
+ * stage's epilogue and the next stage's x inverse in one kernel, so the
+ * stage inputs a, b, c never travel through HBM.  Results agree with the
+ * unfused passes to rounding (the inverse runs its radix passes in reverse
+ * order). */
+int vk_solver_set_fusion(vk_solver *s, int enable);
 /* Domain getters (domain.rs:43-68). */
 int vk_solver_time(vk_solver *s, double *time);
 int vk_solver_dt(vk_solver *s, double *dt);

@@ -144,3 +144,28 @@ def test_mixed_precision_tracks_fp64(hip_device, n):
         assert abs(zmx - z64) < 1e-6 * z64
     assert _rel(sol["mixed"].u_hat(), sol["f64"].u_hat()) < 1e-5
     assert sol["mixed"].step_count() == 100
+
+
+@pytest.mark.parametrize("n", [16, 32])
+def test_fused_x_pass_matches_unfused_and_oracle(hip_device, n):
+    """The fused x pass (stage epilogue + next stage's x inverse in one kernel,
+    the stage inputs never stored) against the unfused passes and the oracle:
+    adaptive dt, 4 steps, FP64 and mixed."""
+    from vonkarman_amd import GridSpec, IcType, Periodic3D
+
+    print(f"case fused vs unfused n={n}", flush=True)
+    u0 = _random_state(n, 7 + n) * 1e-2
+    nu = 0.01
+    r = O.Solver(n, nu)
+    r.set_state(u0)
+    r.init_dt()
+    r.step(4)
+    for prec, tol in (("f64", 1e-13), ("mixed", 1e-5)):
+        out = {}
+        for fuse in (True, False):
+            s = Periodic3D(GridSpec.cubic(n), nu, IcType.Spectral(u0), precision=prec)
+            s.set_fusion(fuse)
+            s.steps(4)
+            out[fuse] = s.u_hat()
+        assert _rel(out[True], out[False]) < tol, prec
+        assert _rel(out[True], r.state()) < (1e-12 if prec == "f64" else 1e-5), prec

@@ -55,7 +55,8 @@ class VkDiag(ctypes.Structure):
 
 
 class VkTiming(ctypes.Structure):
-    """``vk_timing``: per-pass device milliseconds of vk_solver_step_timed."""
+    """``vk_timing``: per-pass device milliseconds of vk_solver_step_timed,
+    with the number of launches summed in each x-pass field."""
 
     _fields_ = [
         ("x_inverse", ctypes.c_double),
@@ -66,6 +67,10 @@ class VkTiming(ctypes.Structure):
         ("cfl", ctypes.c_double),
         ("total", ctypes.c_double),
         ("steps", ctypes.c_int),
+        ("n_x_inverse", ctypes.c_int),
+        ("n_x_forward", ctypes.c_int),
+        ("n_x_fused", ctypes.c_int),
+        ("x_fused", ctypes.c_double),
     ]
 
 
@@ -93,6 +98,7 @@ SIGNATURES = {
     "vk_solver_set_clock": (_I, [_P, _D, _U64]),
     "vk_solver_step": (_I, [_P, _I]),
     "vk_solver_step_timed": (_I, [_P, _I, ctypes.POINTER(VkTiming)]),
+    "vk_solver_set_fusion": (_I, [_P, _I]),
     "vk_solver_time": (_I, [_P, _PD]),
     "vk_solver_dt": (_I, [_P, _PD]),
     "vk_solver_step_count": (_I, [_P, _PU64]),

@@ -297,6 +297,7 @@ struct vk_solver {
     double l, ks, nu, dx;
     Prec wp;
     bool fixed = false;
+    bool fuse_x = true;  // stage epilogue + next x inverse in one pass
     hipStream_t st = nullptr;
     long long S;  // complex elements per state field (n*n*hp)
     DevBuf u, x, n1, sacc;  // state-precision (FP64) arrays [3][S]
@@ -328,7 +329,13 @@ namespace {
 
 // one nonlinear evaluation N = P trunc FFT(u x curl u) of the state-layout
 // field `src`, fused with the ETD-RK4 stage epilogue `sa`.
-int nl_eval(vk_solver *s, const void *src, StageArgs &sa, vk_timing *t, hipEvent_t *ev) {
+//   skip_p1: A already holds the x inverse of src (the previous evaluation's
+//            stage pass produced it);
+//   fuse:    the stage pass also runs the x inverse of its stage output into A
+//            for the next evaluation (sa.out, if set, still receives the stage
+//            output).
+int nl_eval(vk_solver *s, const void *src, StageArgs &sa, vk_timing *t, hipEvent_t *ev, bool skip_p1 = false,
+            bool fuse = false) {
     const int N = s->n, M = s->m, Hp = s->hp, H = s->h;
     const long long NMH = (long long)N * M * Hp, MMH = (long long)M * M * Hp, NH = (long long)N * Hp;
     const long long MH = (long long)M * Hp;
@@ -339,7 +346,8 @@ int nl_eval(vk_solver *s, const void *src, StageArgs &sa, vk_timing *t, hipEvent
     if (int r = mark(0)) return r;
     // P1: x inverse with padding: u[3][N][N][Hp] -> A[5][N][M][Hp] (y-major)
     Strided g1 = make_strided((int)NH, Hp, H, (int)NH, 0, NH, s->S, Hp, MH, Hp, NMH);
-    VK_TRY(launch_inv(INV_U3DX, M, N, s->wp, s->wp, src, s->A.p, s->tw_m.p, g1, N, s->ks, 1.0, s->st));
+    if (!skip_p1)
+        VK_TRY(launch_inv(INV_U3DX, M, N, s->wp, s->wp, src, s->A.p, s->tw_m.p, g1, N, s->ks, 1.0, s->st));
     if (int r = mark(1)) return r;
     // P2: y inverse with padding, curl completed: A -> B[6][M][M][Hp]
     Strided g2 = make_strided((int)MH, Hp, H, (int)MH, 0, MH, NMH, Hp, MH, Hp, MMH);
@@ -355,8 +363,12 @@ int nl_eval(vk_solver *s, const void *src, StageArgs &sa, vk_timing *t, hipEvent
     VK_TRY(launch_fwd(FWD_PLAIN, M, N, 3, s->wp, s->A.p, s->B.p, s->tw_m.p, g3, N, s->ks, 1.0, nullptr, s->st));
     if (int r = mark(4)) return r;
     // P1': x forward, truncate, Leray, stage epilogue: D (B) -> state arrays
+    // (fused: and the next evaluation's P1 of the stage output -> A)
     Strided g4 = make_strided((int)NH, Hp, H, Hp, MH, Hp, NMH, (int)NH, 0, NH, s->S);
-    VK_TRY(launch_fwd(FWD_STAGE, M, N, 3, s->wp, s->B.p, nullptr, s->tw_m.p, g4, N, s->ks, 1.0, &sa, s->st));
+    if (fuse)
+        VK_TRY(launch_fwd_inv(M, N, s->wp, s->B.p, s->A.p, s->tw_m.p, g4, g1, N, s->ks, &sa, s->st));
+    else
+        VK_TRY(launch_fwd(FWD_STAGE, M, N, 3, s->wp, s->B.p, nullptr, s->tw_m.p, g4, N, s->ks, 1.0, &sa, s->st));
     if (int r = mark(5)) return r;
     return 0;
 }
@@ -424,17 +436,25 @@ int one_step(vk_solver *s, vk_timing *t) {
     }
     static const int stages[4] = {ST_1, ST_2, ST_3, ST_4};
     for (int k = 0; k < 4; ++k) {
+        // fused: stages 1-3 hand the next stage input to the next evaluation
+        // through its x inverse in A; the stage inputs are not stored
+        const bool skip = s->fuse_x && k > 0, fuse = s->fuse_x && k < 3;
         const void *src = k == 0 ? s->u.p : s->x.p;
-        void *dst = k == 3 ? s->u.p : s->x.p;
+        void *dst = k == 3 ? s->u.p : (fuse ? nullptr : s->x.p);
         StageArgs sa = stage_args(s, stages[k], dst);
-        if (int r = nl_eval(s, src, sa, t, s->ev)) return r;
+        if (int r = nl_eval(s, src, sa, t, s->ev, skip, fuse)) return r;
         if (t) {
             VK_HIP(hipEventSynchronize(s->ev[5]));
-            double *acc[5] = {&t->x_inverse, &t->y_inverse, &t->z_cross, &t->y_forward, &t->x_forward};
+            double *acc[5] = {&t->x_inverse, &t->y_inverse, &t->z_cross, &t->y_forward,
+                              fuse ? &t->x_fused : &t->x_forward};
             for (int i = 0; i < 5; ++i) {
+                if (i == 0 && skip) continue;
                 VK_HIP(hipEventElapsedTime(&ms, s->ev[i], s->ev[i + 1]));
                 *acc[i] += ms;
             }
+            t->n_x_inverse += skip ? 0 : 1;
+            t->n_x_forward += fuse ? 0 : 1;
+            t->n_x_fused += fuse ? 1 : 0;
         }
     }
     VK_TRY(launch_clock_advance(s->dsc(), s->st));
@@ -458,7 +478,11 @@ int vk_solver_create(int n, double l, double nu, int precision, vk_solver **out)
     s->m = 3 * n / 2;  // GridSpec::padded_3half (vonkarman-core/src/field.rs:53-62)
     s->h = n / 2 + 1;
     s->wp = precision == VK_PREC_F64 ? F64 : F32;
-    const int row_elems = s->wp == F64 ? 8 : 16;  // one 128-byte row per tile position
+#ifndef VK_HP_ALIGN_BYTES
+#define VK_HP_ALIGN_BYTES 128
+#endif
+    // spectral rows padded to whole tile rows (VK_HP_ALIGN_BYTES per row)
+    const int row_elems = VK_HP_ALIGN_BYTES / (s->wp == F64 ? 16 : 8);
     s->hp = round_up(s->h, row_elems);
     s->l = l;
     s->ks = 2.0 * 3.14159265358979323846 / l;  // spectral_ops.rs:52
@@ -607,6 +631,12 @@ int vk_solver_step_timed(vk_solver *s, int nsteps, vk_timing *t) {
     t->steps += nsteps;
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
+    return VK_OK;
+}
+
+int vk_solver_set_fusion(vk_solver *s, int enable) {
+    if (int r = check(s)) return r;
+    s->fuse_x = enable != 0;
     return VK_OK;
 }
 
@@ -834,12 +864,15 @@ int vjp_nl(vk_solver *s, const void *v, StageArgs &sa) {
 int step_vjp(vk_solver *s) {
     if (int r = ensure_vjp(s)) return r;
     // forward recompute of the stage inputs a, b, c (solver.rs:253-343)
+    // (with the forward step's pass fusion, so a and b are computed by the
+    // same kernels as in the step; they are also stored for the reverse sweep)
+    const bool fz = s->fuse_x;
     StageArgs f1 = stage_args(s, ST_1, s->sa_.p);
-    if (int r = nl_eval(s, s->u.p, f1, nullptr, nullptr)) return r;
+    if (int r = nl_eval(s, s->u.p, f1, nullptr, nullptr, false, fz)) return r;
     StageArgs f2 = stage_args(s, ST_2, s->sb_.p);
-    if (int r = nl_eval(s, s->sa_.p, f2, nullptr, nullptr)) return r;
+    if (int r = nl_eval(s, s->sa_.p, f2, nullptr, nullptr, fz, fz)) return r;
     StageArgs f3 = stage_args(s, ST_3, s->sc_.p);
-    if (int r = nl_eval(s, s->sb_.p, f3, nullptr, nullptr)) return r;
+    if (int r = nl_eval(s, s->sb_.p, f3, nullptr, nullptr, fz, false)) return r;
     // reverse sweep
     StageArgs tmp = adj_args(s, AS_4, nullptr);
     VK_TRY(launch_adj_prep(static_cast<const double *>(s->lam.p), static_cast<double *>(s->cot.p), s->n, s->hp,

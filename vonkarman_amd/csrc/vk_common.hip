@@ -22,6 +22,13 @@ int launch_fwd(int mode, int L, int lout, int nf, Prec wp, const void *in, void 
     switch (L) { VK_FOR_EACH_LENGTH(VK_CASE_FWD) }
     return 2;
 }
+#define VK_CASE_FWDINV(LEN) \
+    case LEN: return launch_fwd_inv_L<LEN>(lout, wp, in, aout, tw, g, gi, n, ks, sa, st);
+int launch_fwd_inv(int L, int lout, Prec wp, const void *in, void *aout, const void *tw, const Strided &g,
+                   const Strided &gi, int n, double ks, const StageArgs *sa, hipStream_t st) {
+    switch (L) { VK_FOR_EACH_LENGTH(VK_CASE_FWDINV) }
+    return 2;
+}
 #define VK_CASE_ZNL(LEN) \
     case LEN: return launch_z_nl_L<LEN>(wp, in, out, lines, hp, fs_in, fs_out, s_in, s_out, tw, st);
 int launch_z_nl(int L, Prec wp, const void *in, void *out, long long lines, int hp, long long fs_in,

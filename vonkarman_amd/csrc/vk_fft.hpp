@@ -418,7 +418,13 @@ struct PlanSmall<L, false> {
 // position adjacent (strided tiles), NT = T*L/E threads (>= 64)
 template <int L, int E, typename R, int ROW_BYTES = 128>
 struct Geom {
-    static constexpr int T0 = ROW_BYTES / (2 * (int)sizeof(R));
+    // columns per row, halved while the LDS tile would exceed 100 KB
+    static constexpr int tcols() {
+        int t = ROW_BYTES / (2 * (int)sizeof(R));
+        while (t > 1 && L * t * 2 * (int)sizeof(R) > 100 * 1024) t /= 2;
+        return t;
+    }
+    static constexpr int T0 = tcols();
     static constexpr int TMIN = (64 * E + L - 1) / L;
     static constexpr int T = T0 >= TMIN ? T0 : ((TMIN + T0 - 1) / T0) * T0;
     static constexpr int NT = T * L / E;

@@ -11,6 +11,8 @@ template int launch_inv_L<VK_L>(int, int, Prec, Prec, const void *, void *, cons
                                 double, double, hipStream_t);
 template int launch_fwd_L<VK_L>(int, int, int, Prec, const void *, void *, const void *, const Strided &, int,
                                 double, double, const StageArgs *, hipStream_t);
+template int launch_fwd_inv_L<VK_L>(int, Prec, const void *, void *, const void *, const Strided &, const Strided &,
+                                    int, double, const StageArgs *, hipStream_t);
 template int launch_z_nl_L<VK_L>(Prec, const void *, void *, long long, int, long long, long long, double, double,
                                  const void *, hipStream_t);
 template int launch_z_vjp_L<VK_L>(Prec, const void *, void *, long long, int, long long, long long, double, double,

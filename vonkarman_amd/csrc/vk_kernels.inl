@@ -249,9 +249,12 @@ __global__ __launch_bounds__((InvGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
 // ETD-RK4 stage epilogue (vonkarman-periodic/src/solver.rs:264-378) for CH
 // modes; nn = Leray-projected N-hat of the modes.  Table rows: 0 e, 1 e_half,
 // 2 a21 (= a31), 3 a41, 4 b1, 5 b23, 6 b4.
-template <int STG, int CH, typename TS>
+// XO: also return the stage output (the next stage input, or the new state)
+// in xo; it is stored to sa.out only when sa.out is set (uniform branch).
+template <int STG, int CH, typename TS, bool XO = false>
 __device__ __forceinline__ void stage_chunk(const StageArgs &sa, bool st, const long long (&idx)[CH],
-                                            const int (&q)[CH], const cpx<TS> (&nn)[CH][3]) {
+                                            const int (&q)[CH], const cpx<TS> (&nn)[CH][3],
+                                            cpx<TS> (*xo)[3] = nullptr) {
     cpx<TS> *out = reinterpret_cast<cpx<TS> *>(sa.out);
     const cpx<TS> *u = reinterpret_cast<const cpx<TS> *>(sa.u);
     cpx<TS> *n1 = reinterpret_cast<cpx<TS> *>(sa.n1);
@@ -284,10 +287,17 @@ __device__ __forceinline__ void stage_chunk(const StageArgs &sa, bool st, const 
                 for (int f = 0; f < 3; ++f) {
                     const cpx<TS> x = nn[i][f];
                     (STG == ST_1 ? n1 : sacc)[f * fs + idx[i]] = x;
-                    out[f * fs + idx[i]] = {TS(eh[i] * uu[i][f].re + dt * a[i] * x.re),
-                                            TS(eh[i] * uu[i][f].im + dt * a[i] * x.im)};
                 }
         }
+#pragma unroll
+        for (int i = 0; i < CH; ++i)
+#pragma unroll
+            for (int f = 0; f < 3; ++f) {
+                const cpx<TS> x = nn[i][f];
+                const cpx<TS> y = {TS(eh[i] * uu[i][f].re + dt * a[i] * x.re), TS(eh[i] * uu[i][f].im + dt * a[i] * x.im)};
+                if constexpr (XO) xo[i][f] = y;
+                if (st && (!XO || out)) out[f * fs + idx[i]] = y;
+            }
     } else if constexpr (STG == ST_3) {
         const double dt = sa.sc->dt;
         double e[CH], a[CH];
@@ -310,11 +320,18 @@ __device__ __forceinline__ void stage_chunk(const StageArgs &sa, bool st, const 
                 for (int f = 0; f < 3; ++f) {
                     const cpx<TS> x = nn[i][f];
                     sacc[f * fs + idx[i]] = {s2[i][f].re + x.re, s2[i][f].im + x.im};
-                    const double dr = 2.0 * x.re - m1[i][f].re, di = 2.0 * x.im - m1[i][f].im;
-                    out[f * fs + idx[i]] = {TS(e[i] * uu[i][f].re + dt * a[i] * dr),
-                                            TS(e[i] * uu[i][f].im + dt * a[i] * di)};
                 }
         }
+#pragma unroll
+        for (int i = 0; i < CH; ++i)
+#pragma unroll
+            for (int f = 0; f < 3; ++f) {
+                const cpx<TS> x = nn[i][f];
+                const double dr = 2.0 * x.re - m1[i][f].re, di = 2.0 * x.im - m1[i][f].im;
+                const cpx<TS> y = {TS(e[i] * uu[i][f].re + dt * a[i] * dr), TS(e[i] * uu[i][f].im + dt * a[i] * di)};
+                if constexpr (XO) xo[i][f] = y;
+                if (st && (!XO || out)) out[f * fs + idx[i]] = y;
+            }
     } else {  // ST_4: final combine into u
         static_assert(STG == ST_4, "unknown stage");
         const double dt = sa.sc->dt;
@@ -333,17 +350,17 @@ __device__ __forceinline__ void stage_chunk(const StageArgs &sa, bool st, const 
                 s[i][f] = sacc[f * fs + idx[i]];
             }
         }
-        if (st) {
 #pragma unroll
-            for (int i = 0; i < CH; ++i)
+        for (int i = 0; i < CH; ++i)
 #pragma unroll
-                for (int f = 0; f < 3; ++f) {
-                    const cpx<TS> x = nn[i][f];
-                    const double rr = b1[i] * m1[i][f].re + b23[i] * s[i][f].re + b4[i] * x.re;
-                    const double ri = b1[i] * m1[i][f].im + b23[i] * s[i][f].im + b4[i] * x.im;
-                    out[f * fs + idx[i]] = {TS(e[i] * uu[i][f].re + dt * rr), TS(e[i] * uu[i][f].im + dt * ri)};
-                }
-        }
+            for (int f = 0; f < 3; ++f) {
+                const cpx<TS> x = nn[i][f];
+                const double rr = b1[i] * m1[i][f].re + b23[i] * s[i][f].re + b4[i] * x.re;
+                const double ri = b1[i] * m1[i][f].im + b23[i] * s[i][f].im + b4[i] * x.im;
+                const cpx<TS> y = {TS(e[i] * uu[i][f].re + dt * rr), TS(e[i] * uu[i][f].im + dt * ri)};
+                if constexpr (XO) xo[i][f] = y;
+                if (st && (!XO || out)) out[f * fs + idx[i]] = y;
+            }
     }
 }
 
@@ -680,6 +697,171 @@ __global__ __launch_bounds__((FwdGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
             __syncthreads();
         }
     }
+}
+
+// ===========================================================================
+// x forward + truncation + Leray + ETD-RK4 stage epilogue + the next stage's
+// x inverse (pad + i kx terms), fused in one pass over the same columns (y, z):
+// the stage output (the next stage input) is stored only when sa.out is set.
+// The forward plan leaves element e at position j + m L/RP, which is where
+// the reversed plan's first pass expects element e, and 3/2 truncation
+// followed by zero padding keeps exactly those positions, so the kept stage
+// outputs feed the inverse from registers with zeros in the gap elements.
+// Outputs (5 fields, the x pass layout of the next nonlinear evaluation):
+// 0 ux, 1 uy, 2 uz, 3 i kx uz, 4 i kx uy.
+// ===========================================================================
+template <int L, int LOUT, int STG, typename TW>
+__global__ __launch_bounds__((FwdGeom<L, FWD_STAGE, TW>::NT)) __attribute__((amdgpu_waves_per_eu(Waves<K_FWD, TW>::value))) void k_fwd_stage_inv(
+    const cpx<TW> *__restrict__ in, cpx<TW> *__restrict__ aout, const cpx<TW> *__restrict__ tw, Strided g, Strided gi,
+    int n, double ks, StageArgs sa) {
+    using P = PlanE12<L>;
+    using PL = typename P::radices;         // forward order
+    using RL = typename Reverse<PL>::type;  // inverse order, first radix = forward's last
+    constexpr int E = P::E;
+    using G = FwdGeom<L, FWD_STAGE, TW>;
+    constexpr int T = G::T, NT = G::NT;
+    static_assert(T == InvGeom<L, INV_U3DX, TW>::T && NT == InvGeom<L, INV_U3DX, TW>::NT,
+                  "the fused x passes share one tile");
+    constexpr int RP = PL::last;
+    constexpr int KR = 2 * RP / 3, K = (E / RP) * KR;
+    static_assert(RP % 3 == 0 && 3 * LOUT == 2 * L, "stage pass needs a 3/2-padded plan");
+    constexpr bool HEAVY = STG == ST_3 || STG == ST_4;
+    constexpr int CH = sizeof(TW) == 8 ? (HEAVY ? 1 : 2) : (HEAVY ? 2 : 4);
+    static_assert(K % CH == 0, "chunks must tile the kept modes");
+    __shared__ cpx<TW> lds[L * T];
+    __shared__ cpx<TW> ltw[L];
+    const int tid = threadIdx.x;
+    const int col0 = blockIdx.x * T;
+    cpx<TW> v[E], Q[E];
+    cpx<TW> r[3][K];
+    stage_twiddles<L, NT>(ltw, tw, tid);
+
+    const int col = col0 + tid % T;
+    const bool colok = col < g.ncols && (col % g.hp) < g.hvalid;
+    const long long cin = colok ? col_addr(col, g.in_inner, g.in_os) : 0;
+    const long long cout = colok ? col_addr(col, g.out_inner, g.out_os) : 0;
+    const long long cinv = colok ? col_addr(col, gi.out_inner, gi.out_os) : 0;
+    auto load_into = [&](cpx<TW>(&S)[E], int f) {
+        const gptr<cpx<TW>> src = opaque(in) + f * g.in_fs + cin;
+#pragma unroll
+        for (int e = 0; e < E; ++e) S[e] = ldg(src, (long long)ci_pos_in<L, PL::first, T, NT>(tid, e) * g.in_ls);
+    };
+    // forward transforms of the three fields (each next field loads meanwhile)
+    load_into(v, 0);
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+        if (f + 1 < 3) load_into(Q, f + 1);
+        fft_reg_lt<L, -1, T, NT, T, 1, PL>(v, lds, ltw, tid);
+#pragma unroll
+        for (int b = 0; b < E / RP; ++b)
+#pragma unroll
+            for (int mk = 0; mk < KR; ++mk) r[f][b * KR + mk] = v[b * RP + (mk < RP / 3 ? mk : mk + RP / 3)];
+        __syncthreads();
+        if (f + 1 < 3) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) v[e] = Q[e];
+        }
+    }
+    // Leray + stage epilogue per kept mode; r <- the stage output
+#pragma unroll
+    for (int c0 = 0; c0 < K; c0 += CH) {
+        long long idx[CH];
+        int q[CH];
+        cpx<TW> nn[CH][3], xo[CH][3];
+#pragma unroll
+        for (int i = 0; i < CH; ++i) {
+            const int b = (c0 + i) / KR, mk = (c0 + i) % KR;
+            const int p = ci_pos_out<L, RP, T, NT>(tid, b * RP + (mk < RP / 3 ? mk : mk + RP / 3));
+            const int j = unpad_index(p, LOUT, L);
+            const int iy = col / g.hp, iz = col % g.hp;
+            const int wx = wavenum(j, n), wy = wavenum(iy, n);
+            idx[i] = cout + (long long)j * g.out_ls;
+            q[i] = colok ? wx * wx + wy * wy + iz * iz : 0;
+            const TW kx = TW(double(wx) * ks), ky = TW(double(wy) * ks), kz = TW(double(iz) * ks);
+            // Leray projection (vonkarman-core/src/spectral_ops.rs:114-151)
+            cpx<TW> x = r[0][c0 + i], y = r[1][c0 + i], z = r[2][c0 + i];
+            const TW k2 = kx * kx + ky * ky + kz * kz;
+            if (!(double(k2) < 1e-30)) {
+                const TW inv = TW(1) / k2;
+                const TW dr = kx * x.re + ky * y.re + kz * z.re;
+                const TW di = kx * x.im + ky * y.im + kz * z.im;
+                x = {x.re - kx * dr * inv, x.im - kx * di * inv};
+                y = {y.re - ky * dr * inv, y.im - ky * di * inv};
+                z = {z.re - kz * dr * inv, z.im - kz * di * inv};
+            }
+            nn[i][0] = x;
+            nn[i][1] = y;
+            nn[i][2] = z;
+        }
+        stage_chunk<STG, CH, TW, true>(sa, colok, idx, q, nn, xo);
+#pragma unroll
+        for (int i = 0; i < CH; ++i)
+#pragma unroll
+            for (int f = 0; f < 3; ++f) r[f][c0 + i] = xo[i][f];
+    }
+    // x inverse of the stage output: element e of the reversed plan's input is
+    // forward output element e (zero in the truncation gap)
+    auto xin = [&](int f, int e) -> cpx<TW> {
+        const int b = e / RP, m = e % RP;
+        if (m >= RP / 3 && m < 2 * RP / 3) return {TW(0), TW(0)};
+        return r[f][b * KR + (m < RP / 3 ? m : m - RP / 3)];
+    };
+    auto kx_of = [&](int e) -> TW {
+        const int j = unpad_index(ci_pos_out<L, RP, T, NT>(tid, e), LOUT, L);
+        return TW(double(wavenum(j < 0 ? 0 : j, n)) * ks);
+    };
+    auto ik = [](TW k, cpx<TW> a) -> cpx<TW> { return {-k * a.im, k * a.re}; };
+    auto emit = [&](int fo, auto value) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[e] = value(e);
+        fft_reg_lt<L, +1, T, NT, T, 1, RL>(v, lds, ltw, tid);
+        if (colok) {
+#pragma unroll
+            for (int e = 0; e < E; ++e)
+                aout[fo * gi.out_fs + cinv + (long long)ci_pos_out<L, RL::last, T, NT>(tid, e) * gi.out_ls] = v[e];
+        }
+        __syncthreads();
+    };
+    emit(0, [&](int e) { return xin(0, e); });
+    emit(1, [&](int e) { return xin(1, e); });
+    emit(4, [&](int e) { return ik(kx_of(e), xin(1, e)); });
+    emit(2, [&](int e) { return xin(2, e); });
+    emit(3, [&](int e) { return ik(kx_of(e), xin(2, e)); });
+}
+
+template <int L, int LOUT, int STG, typename TW>
+int run_fwd_inv(const void *in, void *aout, const void *tw, const Strided &g, const Strided &gi, int n, double ks,
+                const StageArgs *sa, hipStream_t st) {
+    using C = FwdGeom<L, FWD_STAGE, TW>;
+    const unsigned nb = (unsigned)((g.ncols + C::T - 1) / C::T);
+    hipLaunchKernelGGL((k_fwd_stage_inv<L, LOUT, STG, TW>), dim3(nb), dim3(C::NT), 0, st,
+                       reinterpret_cast<const cpx<TW> *>(in), reinterpret_cast<cpx<TW> *>(aout),
+                       reinterpret_cast<const cpx<TW> *>(tw), g, gi, n, ks, *sa);
+    return hipGetLastError() == hipSuccess ? 0 : 3;
+}
+
+template <int L>
+int launch_fwd_inv_L(int lout, Prec wp, const void *in, void *aout, const void *tw, const Strided &g,
+                     const Strided &gi, int n, double ks, const StageArgs *sa, hipStream_t st) {
+    if constexpr (L % 3 == 0 && (2 * L / 3) % 2 == 0) {
+        constexpr int LOUT = 2 * L / 3;
+        if (lout != LOUT || !sa) return 2;
+        switch (sa->stage) {
+            case ST_1:
+                return wp == F64 ? run_fwd_inv<L, LOUT, ST_1, double>(in, aout, tw, g, gi, n, ks, sa, st)
+                                 : run_fwd_inv<L, LOUT, ST_1, float>(in, aout, tw, g, gi, n, ks, sa, st);
+            case ST_2:
+                return wp == F64 ? run_fwd_inv<L, LOUT, ST_2, double>(in, aout, tw, g, gi, n, ks, sa, st)
+                                 : run_fwd_inv<L, LOUT, ST_2, float>(in, aout, tw, g, gi, n, ks, sa, st);
+            case ST_3:
+                return wp == F64 ? run_fwd_inv<L, LOUT, ST_3, double>(in, aout, tw, g, gi, n, ks, sa, st)
+                                 : run_fwd_inv<L, LOUT, ST_3, float>(in, aout, tw, g, gi, n, ks, sa, st);
+            case ST_4:
+                return wp == F64 ? run_fwd_inv<L, LOUT, ST_4, double>(in, aout, tw, g, gi, n, ks, sa, st)
+                                 : run_fwd_inv<L, LOUT, ST_4, float>(in, aout, tw, g, gi, n, ks, sa, st);
+        }
+    }
+    return 2;
 }
 
 // ===========================================================================

@@ -83,6 +83,12 @@ int launch_fwd(int mode, int L, int lout, int nf, Prec work_prec, const void *in
                const void *tw, const Strided &g, int n, double ks, double scale,
                const StageArgs *sa, hipStream_t st);
 
+// x forward + stage epilogue (ST_1..ST_4) + x inverse of the stage output
+// (INV_U3DX layout, geometry gi) in one pass; the stage output is also
+// stored to sa->out when that is set
+int launch_fwd_inv(int L, int lout, Prec work_prec, const void *in, void *aout, const void *tw, const Strided &g,
+                   const Strided &gi, int n, double ks, const StageArgs *sa, hipStream_t st);
+
 // contiguous (z) passes -------------------------------------------------------
 // NL z pass: 6 padded half-spectra (ux,uy,uz,wx,wy,wz) -> 3 truncated spectra of u x w
 int launch_z_nl(int L, Prec work_prec, const void *in, void *out, long long lines, int hp,

@@ -12,6 +12,9 @@ template <int L>
 int launch_fwd_L(int mode, int lout, int nf, Prec wp, const void *in, void *out, const void *tw,
                  const Strided &g, int n, double ks, double scale, const StageArgs *sa, hipStream_t st);
 template <int L>
+int launch_fwd_inv_L(int lout, Prec wp, const void *in, void *aout, const void *tw, const Strided &g,
+                     const Strided &gi, int n, double ks, const StageArgs *sa, hipStream_t st);
+template <int L>
 int launch_z_nl_L(Prec wp, const void *in, void *out, long long lines, int hp, long long fs_in,
                   long long fs_out, double s_in, double s_out, const void *tw, hipStream_t st);
 template <int L>

@@ -187,6 +187,12 @@ class Periodic3D:
     def steps(self, nsteps: int) -> None:
         check(lib().vk_solver_step(self._h, nsteps), "vk_solver_step")
 
+    def set_fusion(self, enable: bool) -> None:
+        """Fused x pass (stage epilogue + the next stage's x inverse in one
+        kernel; on by default).  A performance switch of this backend with no
+        reference counterpart: results agree with the unfused passes to rounding."""
+        check(lib().vk_solver_set_fusion(self._h, 1 if enable else 0), "vk_solver_set_fusion")
+
     def steps_timed(self, nsteps: int) -> VkTiming:
         t = VkTiming()
         check(lib().vk_solver_step_timed(self._h, nsteps, ctypes.byref(t)), "vk_solver_step_timed")
