@@ -929,19 +929,22 @@ __device__ __forceinline__ void z_pair_fill(cpx<TW> (&v)[E], gptr<cpx<TW>> fa, g
 }
 
 // z tiles are column-interleaved too: the T lines of a block are the columns,
-// so position p of all T lines is one contiguous LDS row.  32-byte rows make
-// a block of one wavefront (T = 2 FP64 lines, NT = 64): its barriers cost
-// almost nothing and blocks on a SIMD run independently.  Each line still
-// reads HBM in runs of 32 consecutive positions per load instruction.
-// Measured at 384 (256^3 solver), per launch: FP64 1.03 ms with 64-byte rows
-// -> 0.87 ms with 32-byte rows; FP32 0.71 -> 0.65 ms.
+// so position p of all T lines is one contiguous LDS row.  Two lines per block
+// make a block of one wavefront (E = 12, NT = 64): the compiler drops its
+// barriers, and blocks on a SIMD run independently.  Each line still reads
+// HBM in runs of 32 consecutive positions per load instruction.  Measured at
+// 384 (256^3 solver), per launch: FP64 1.03 ms with 4 lines per block -> 0.87
+// ms with 2; FP32 0.65 ms with 4 -> 0.50 ms with 2.
 #ifndef VK_Z_ROW_BYTES
-#define VK_Z_ROW_BYTES 32  // T = 2 (FP64) or 4 (FP32) lines per block
+#define VK_Z_ROW_BYTES 32  // T = 2 FP64 lines per block
+#endif
+#ifndef VK_Z_ROW_BYTES_F32
+#define VK_Z_ROW_BYTES_F32 16  // T = 2 FP32 lines per block
 #endif
 template <int L, typename TW>
 using ZNLP = PlanE12<L>;
 template <int L, typename TW>
-using ZNLGeom = Geom<L, ZNLP<L, TW>::E, TW, VK_Z_ROW_BYTES>;
+using ZNLGeom = Geom<L, ZNLP<L, TW>::E, TW, sizeof(TW) == 8 ? VK_Z_ROW_BYTES : VK_Z_ROW_BYTES_F32>;
 
 template <int L, typename TW>
 __global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) __attribute__((amdgpu_waves_per_eu(Waves<K_ZNL, TW>::value))) void k_z_nl(const cpx<TW> *__restrict__ in,
