@@ -187,11 +187,20 @@ def adjoint_cost(n: int, nsteps: int = 8) -> dict:
                                        ctypes.byref(fs)), "vk_solver_adjoint")
         t = time.perf_counter() - t0
         key = "all_states" if snaps > nsteps else "two_snapshots"
+        step_fwd = t_fwd / nsteps
+        # one reverse step (stage recompute + 4 nonlinear VJPs) against one
+        # forward step: the gradient's time less its forward steps, per step
+        rev_step = (t - fs.value * step_fwd) / nsteps
         res[key] = {"snapshots": min(snaps, nsteps), "adjoint_s": t, "forward_steps": fs.value,
-                    "ratio_vs_forward": t / t_fwd}
+                    "gradient_over_forward_trajectory": t / t_fwd,
+                    "reverse_step_ms": 1e3 * rev_step, "reverse_step_over_forward_step": rev_step / step_fwd}
+    res["forward_step_ms"] = 1e3 * t_fwd / nsteps
     res["device_bytes"] = s.device_bytes()
-    print(f"[bench] adjoint: {nsteps} steps, ratio {res['all_states']['ratio_vs_forward']:.2f} "
-          f"(two snapshots {res['two_snapshots']['ratio_vs_forward']:.2f})", file=sys.stderr, flush=True)
+    print(f"[bench] adjoint: {nsteps} steps, reverse step / forward step "
+          f"{res['all_states']['reverse_step_over_forward_step']:.2f}, gradient / forward trajectory "
+          f"{res['all_states']['gradient_over_forward_trajectory']:.2f} "
+          f"(two snapshots {res['two_snapshots']['gradient_over_forward_trajectory']:.2f})", file=sys.stderr,
+          flush=True)
     del s
     return res
 
@@ -255,6 +264,23 @@ def workload_mixed_error(sizes=(256, 512), nsteps: int = 1000) -> dict:
         print(f"[bench] mixed-error n={n}: {json.dumps(out[str(n)])}", file=sys.stderr, flush=True)
         del sol
     return out
+
+
+def workload_hbm_ceiling() -> dict:
+    """Practical HBM ceilings on this device for the passes' access patterns
+    (vonkarman_amd/lib/hbm_probe, built with the library): a contiguous copy
+    and tiles of T adjacent 16-byte columns at 384 positions, the pattern of
+    the strided x and y passes (T = 8 there, 128-byte rows)."""
+    import subprocess
+
+    exe = os.path.join(ROOT, "vonkarman_amd", "lib", "hbm_probe")
+    if not os.path.exists(exe):
+        raise SystemExit(f"bench.py: {exe} is missing: build with make -C vonkarman_amd/csrc")
+    cp = subprocess.run([exe], stdout=subprocess.PIPE, stderr=subprocess.PIPE, timeout=300, check=True)
+    rows = [json.loads(l) for l in cp.stdout.decode().splitlines() if l.startswith("{")]
+    return {"workload": "HBM ceilings of the pass access patterns (2 GiB buffers, 10 timed launches each)",
+            "peak_GBps": HBM_PEAK_GBS, "patterns": rows,
+            "fraction_of_peak": {r["pattern"]: r["GBps"] / HBM_PEAK_GBS for r in rows}}
 
 
 def _solenoidal_direction(n: int, seed: int):
@@ -546,7 +572,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-traffic", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--workload", default="step", choices=["step", "mixed-error", "adjoint-tubes", "adjoint-iso-mixed"],
+    ap.add_argument("--workload", default="step",
+                    choices=["step", "mixed-error", "adjoint-tubes", "adjoint-iso-mixed", "hbm-ceiling"],
                     help="step: the throughput line (default); the others run the accuracy/adjoint configurations "
                          "and print their own JSON line")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
@@ -560,7 +587,7 @@ def main():
         raise SystemExit(2)
     if args.workload != "step":
         fn = {"mixed-error": workload_mixed_error, "adjoint-tubes": workload_adjoint_tubes,
-              "adjoint-iso-mixed": workload_adjoint_iso_mixed}[args.workload]
+              "adjoint-iso-mixed": workload_adjoint_iso_mixed, "hbm-ceiling": workload_hbm_ceiling}[args.workload]
         print(json.dumps(fn()), flush=True)
         return
 

@@ -136,6 +136,10 @@ int vk_solver_set_clock(vk_solver *s, double time, uint64_t step);
 int vk_solver_step(vk_solver *s, int nsteps);
 /* vk_solver_step with per-pass device timing accumulated into *t. */
 int vk_solver_step_timed(vk_solver *s, int nsteps, vk_timing *t);
+/* rk4_step (vonkarman-periodic/src/rk4.rs:16-146): nsteps classical RK4 steps
+ * of du/dt = N(u) - nu |k|^2 u with step dt on the solver's state (the
+ * validation cross-check of the ETD-RK4 scheme; the clock is not advanced). */
+int vk_solver_rk4_step(vk_solver *s, double dt, int nsteps);
 /* No counterpart in the reference (a performance switch of this backend):
  * enable (1, the default) or disable (0) the fused x pass that runs each
  * stage's epilogue and the next stage's x inverse in one kernel, so the

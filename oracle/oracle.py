@@ -73,6 +73,7 @@ def lib():
             "or_r2c_3d_T": (i, [i, i, i, P, P]),
             "or_nonlinear_vjp": (i, [i, d, P, P, P]),
             "or_solver_step_vjp": (i, [P, P, P]),
+            "or_rk4_step": (i, [i, d, d, d, P, i]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -166,6 +167,14 @@ def nonlinear_vjp(uhat: np.ndarray, lam: np.ndarray, l: float = TWO_PI) -> np.nd
     out = np.empty_like(uhat)
     lib().or_nonlinear_vjp(uhat.shape[1], l, _p(uhat), _p(lam), _p(out))
     return out
+
+
+def rk4_step(uhat: np.ndarray, nu: float, dt: float, nsteps: int = 1, l: float = TWO_PI) -> np.ndarray:
+    """rk4_step (vonkarman-periodic/src/rk4.rs:16-146), nsteps times; returns
+    the advanced state (the input is not modified)."""
+    u = np.array(uhat, dtype=np.complex128, order="C", copy=True)
+    lib().or_rk4_step(u.shape[1], l, nu, dt, _p(u), nsteps)
+    return u
 
 
 def phi1(z: float) -> float:

@@ -661,6 +661,45 @@ extern "C" int or_solver_step(or_solver *s, int nsteps) {
     }
     return 0;
 }
+/* rk4_step (vonkarman-periodic/src/rk4.rs:16-146): classical RK4 on
+ * du/dt = N(u) - nu k^2 u (no integrating factor); the state advances, the
+ * clock does not (the reference's free function takes u_hat alone). */
+extern "C" int or_rk4_step(int n, double l, double nu, double dt, double *uhat, int nsteps) {
+    const Ops o(n, l);
+    const size_t S = o.sz(), T = 3 * S;
+    cd *u = reinterpret_cast<cd *>(uhat);
+    std::vector<cd> k1(T), k2(T), k3(T), k4(T), tmp(T);
+    // compute_rhs (rk4.rs:30-45): rhs = N(state), then rhs += -nu k2 state
+    auto rhs = [&](const cd *state, cd *r) {
+        nonlinear(o, state, r);
+        for (int c = 0; c < 3; ++c)
+            for (size_t i = 0; i < S; ++i) {
+                const double visc = -nu * o.k2[i];
+                const cd st = state[c * S + i];
+                r[c * S + i] = cd(r[c * S + i].real() + visc * st.real(), r[c * S + i].imag() + visc * st.imag());
+            }
+    };
+    for (int step = 0; step < nsteps; ++step) {
+        rhs(u, k1.data());
+        for (size_t j = 0; j < T; ++j)
+            tmp[j] = cd(u[j].real() + 0.5 * dt * k1[j].real(), u[j].imag() + 0.5 * dt * k1[j].imag());
+        rhs(tmp.data(), k2.data());
+        for (size_t j = 0; j < T; ++j)
+            tmp[j] = cd(u[j].real() + 0.5 * dt * k2[j].real(), u[j].imag() + 0.5 * dt * k2[j].imag());
+        rhs(tmp.data(), k3.data());
+        for (size_t j = 0; j < T; ++j)
+            tmp[j] = cd(u[j].real() + dt * k3[j].real(), u[j].imag() + dt * k3[j].imag());
+        rhs(tmp.data(), k4.data());
+        const double sixth_dt = dt / 6.0;
+        for (size_t j = 0; j < T; ++j) {
+            const double sr = k1[j].real() + 2.0 * k2[j].real() + 2.0 * k3[j].real() + k4[j].real();
+            const double si = k1[j].imag() + 2.0 * k2[j].imag() + 2.0 * k3[j].imag() + k4[j].imag();
+            u[j] = cd(u[j].real() + sixth_dt * sr, u[j].imag() + sixth_dt * si);
+        }
+    }
+    return 0;
+}
+
 extern "C" double or_solver_time(const or_solver *s) { return s->time; }
 extern "C" double or_solver_dt(const or_solver *s) { return s->dt; }
 extern "C" uint64_t or_solver_step_count(const or_solver *s) { return s->step; }

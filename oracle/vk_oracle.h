@@ -81,6 +81,11 @@ int or_solver_diagnostics(const or_solver *s, double *out);
 int or_solver_energy_spectrum(const or_solver *s, double *out, int cap);
 int or_energy_spectrum(int n, double l, const double *uhat, double *out, int cap);
 
+/* rk4_step (vonkarman-periodic/src/rk4.rs:16-146): nsteps classical RK4
+ * steps of du/dt = N(u) - nu k^2 u on the interleaved complex [3][n][n][n/2+1]
+ * state, in place. */
+int or_rk4_step(int n, double l, double nu, double dt, double *uhat, int nsteps);
+
 /* --- discrete adjoint (no reference counterpart) ------------------------- */
 /* Transposes under the real inner product over stored entries. */
 int or_c2r_3d_T(int nx, int ny, int nz, const double *g, double *out);

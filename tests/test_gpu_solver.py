@@ -169,3 +169,52 @@ def test_fused_x_pass_matches_unfused_and_oracle(hip_device, n):
             out[fuse] = s.u_hat()
         assert _rel(out[True], out[False]) < tol, prec
         assert _rel(out[True], r.state()) < (1e-12 if prec == "f64" else 1e-5), prec
+
+
+@pytest.mark.parametrize("n", [16, 32])
+def test_rk4_step_matches_oracle(hip_device, n):
+    """rk4_step (rk4.rs:16-146) on the GPU against the oracle: random
+    solenoidal state, 3 steps; and the single-mode degree-4 factor."""
+    import math
+
+    from vonkarman_amd import GridSpec, IcType, Periodic3D
+
+    print(f"case rk4 n={n}", flush=True)
+    nu, dt = 0.02, 2e-3
+    u0 = O.leray(_random_state(n, 31 + n) * 1e-2)
+    s = Periodic3D(GridSpec.cubic(n), nu, IcType.Spectral(u0), init_dt=False)
+    s.rk4_step(dt, 3)
+    assert _rel(s.u_hat(), O.rk4_step(u0, nu, dt, nsteps=3)) < 1e-12
+    assert s.time() == 0.0 and s.step_count() == 0
+    # single shear mode: N = 0, one step multiplies by 1 + z + z^2/2 + z^3/6 + z^4/24
+    u1 = np.zeros_like(u0)
+    u1[0, 0, 2, 1] = 3.0 - 1.0j
+    s.set_u_hat(u1)
+    z = -0.05 * 5.0 * 2.0
+    s2 = Periodic3D(GridSpec.cubic(n), 0.05, IcType.Spectral(u1), init_dt=False)
+    s2.rk4_step(2.0)
+    R = 1 + z + z * z / 2 + z ** 3 / 6 + z ** 4 / 24
+    assert _rel(s2.u_hat(), u1 * R) < 1e-13
+    assert abs(R - math.exp(z)) > 1e-4  # far outside the tolerance: RK4, not the exact decay
+
+
+def test_checkpoint_restart_bitwise(hip_device, tmp_path):
+    """tests/checkpoint_restart.rs: TG n=16, nu=0.01, adaptive dt; 100
+    continuous steps against 50 + write_checkpoint/read_checkpoint + restart
+    + 50: bitwise equal state, equal time and step count."""
+    from vonkarman_amd import GridSpec, IcType, Periodic3D, read_checkpoint, write_checkpoint
+
+    print("case checkpoint restart n=16", flush=True)
+    grid, nu = GridSpec.cubic(16), 0.01
+    cont = Periodic3D(grid, nu, IcType.TaylorGreen())
+    cont.steps(100)
+    first = Periodic3D(grid, nu, IcType.TaylorGreen())
+    first.steps(50)
+    p = str(tmp_path / "checkpoint.npz")
+    write_checkpoint(p, first.checkpoint_data())
+    rest = Periodic3D.from_checkpoint(read_checkpoint(p))
+    rest.steps(50)
+    a, b = cont.u_hat(), rest.u_hat()
+    assert np.array_equal(a.view(np.float64), b.view(np.float64))
+    assert abs(cont.time() - rest.time()) < 1e-14
+    assert cont.step_count() == rest.step_count() == 100

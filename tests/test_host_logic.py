@@ -135,3 +135,35 @@ def test_compute_rates_identity():
     from vonkarman_amd import ScalarDiagnostics
 
     assert ScalarDiagnostics.compute_rates(0.01, 10.0, 5.0) == (-0.2, -0.1)
+
+
+def test_checkpoint_roundtrip_bitwise(tmp_path):
+    """checkpoint.rs:160-210 (checkpoint_roundtrip_bitwise): every field and
+    coefficient survives write + read exactly; the container members carry the
+    reference's HDF5 paths."""
+    from vonkarman_amd import CheckpointData, GridSpec, read_checkpoint, write_checkpoint
+
+    grid = GridSpec.cubic(8)
+    u = np.zeros((3,) + grid.spectral_shape(), np.complex128)
+    for c in range(3):
+        ix, iy, iz = np.meshgrid(*[np.arange(k) for k in grid.spectral_shape()], indexing="ij")
+        u[c] = (ix + iy + iz + c) * 0.123 + 1j * (ix * iy + iz + c) * -0.456
+    data = CheckpointData(u, 1.23456, 12345, 0.00789, grid, 0.01, '[run]\nname = "test"')
+    p = str(tmp_path / "checkpoint.npz")
+    write_checkpoint(p, data)
+    back = read_checkpoint(p)
+    assert back.time == data.time and back.step_count == data.step_count and back.dt == data.dt
+    assert back.nu == data.nu and back.grid == grid and back.config_toml == data.config_toml
+    assert np.array_equal(back.u_hat.view(np.float64), u.view(np.float64))
+    with np.load(p, allow_pickle=False) as z:
+        assert {"metadata/time", "metadata/step_count", "spectral/u_hat_re_0", "spectral/u_hat_im_2",
+                "config/toml"} <= set(z.files)
+        assert z["metadata/step_count"].dtype == np.uint64 and z["spectral/u_hat_re_1"].shape == (8, 8, 5)
+
+
+def test_checkpoint_rejects_wrong_shapes(tmp_path):
+    from vonkarman_amd import CheckpointData, GridSpec, write_checkpoint
+
+    with pytest.raises(ValueError):
+        write_checkpoint(str(tmp_path / "x.npz"),
+                         CheckpointData(np.zeros((3, 8, 8, 4), complex), 0.0, 0, 0.1, GridSpec.cubic(8), 0.01))

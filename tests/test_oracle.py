@@ -404,3 +404,52 @@ def test_fixed_dt_scheme_converges_first_order():
     e1 = np.linalg.norm(run(T / 8) - fine)
     e2 = np.linalg.norm(run(T / 16) - fine)
     assert 1.6 < e1 / e2 < 2.5
+
+
+# ------------------------------------------------------------------------ RK4
+def test_rk4_single_mode_is_the_degree4_taylor_factor():
+    """For a single shear mode N = 0 after the projection (u x w is a
+    gradient), so one classical RK4 step multiplies the state by
+    R(z) = 1 + z + z^2/2 + z^3/6 + z^4/24, z = -nu |k|^2 dt (rk4.rs:16-146):
+    a wrong stage weight or a dropped dt changes R."""
+    n, nu, dt = 8, 0.05, 2.0
+    u = np.zeros((3, n, n, n // 2 + 1), complex)
+    u[0, 0, 2, 1] = 3.0 - 1.0j  # k = (0, 2, 1), |k|^2 = 5
+    z = -nu * 5.0 * dt  # -0.5
+    R = 1 + z + z * z / 2 + z ** 3 / 6 + z ** 4 / 24
+    got = O.rk4_step(u, nu, dt, nsteps=3)
+    assert rel(got, u * R ** 3) < 1e-14
+    assert abs(R ** 3 - math.exp(3 * z)) > 1e-4  # far outside the tolerance: RK4, not the exact decay
+
+
+def test_rk4_energy_decays_tg16():
+    """rk4.rs:155-214: Taylor-Green n=16, nu=0.01, dt=1e-3, 20 steps: energy
+    never increases and ends below 0.999 E0."""
+    n, nu, dt = 16, 0.01, 1e-3
+    u = O.taylor_green(n)
+    uh = np.stack([O.r2c_3d(u[c]) for c in range(3)])
+    w = np.full(n // 2 + 1, 2.0)
+    w[0] = w[n // 2] = 1.0
+
+    def energy(v):
+        return 0.5 * float(np.sum(w * np.abs(v) ** 2)) / float(n) ** 6
+
+    e0 = prev = energy(uh)
+    for _ in range(20):
+        uh = O.rk4_step(uh, nu, dt)
+        e = energy(uh)
+        assert e <= prev + 1e-12 * abs(prev)
+        prev = e
+    assert prev < 0.999 * e0
+
+
+def test_rk4_converges_fourth_order():
+    """Halving dt divides the RK4 error at fixed end time by ~16 (reference
+    solution: RK4 at dt/8), on a random state with the full nonlinear term."""
+    n, nu, T = 8, 0.05, 0.4
+    rng = np.random.default_rng(3)
+    u0 = O.leray((rng.standard_normal((3, n, n, n // 2 + 1)) + 1j * rng.standard_normal((3, n, n, n // 2 + 1))) * 20)
+    ref = O.rk4_step(u0, nu, T / 64, nsteps=64)
+    e1 = rel(O.rk4_step(u0, nu, T / 4, nsteps=4), ref)
+    e2 = rel(O.rk4_step(u0, nu, T / 8, nsteps=8), ref)
+    assert 12.0 < e1 / e2 < 20.0, (e1, e2)

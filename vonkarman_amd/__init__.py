@@ -11,9 +11,11 @@ from .diag import (COLUMNS, DiagnosticsWriter, ScalarDiagnostics, compensated_sp
 from .fft import BackendMode, HipFftBackend, create_backend
 from .grid import GridSpec
 from .periodic import CheckpointData, IcType, Periodic3D, PhysicsParams, Snapshot
+from .checkpoint import read_checkpoint, write_checkpoint
 
 __all__ = [
     "GridSpec", "HipFftBackend", "BackendMode", "create_backend", "Periodic3D", "IcType",
     "PhysicsParams", "CheckpointData", "Snapshot", "ScalarDiagnostics", "energy_spectrum",
-    "compensated_spectrum", "dissipation_spectrum", "DiagnosticsWriter", "COLUMNS",
+    "compensated_spectrum", "dissipation_spectrum", "DiagnosticsWriter", "COLUMNS", "write_checkpoint",
+    "read_checkpoint",
 ]

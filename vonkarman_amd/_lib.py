@@ -99,6 +99,7 @@ SIGNATURES = {
     "vk_solver_step": (_I, [_P, _I]),
     "vk_solver_step_timed": (_I, [_P, _I, ctypes.POINTER(VkTiming)]),
     "vk_solver_set_fusion": (_I, [_P, _I]),
+    "vk_solver_rk4_step": (_I, [_P, _D, _I]),
     "vk_solver_time": (_I, [_P, _PD]),
     "vk_solver_dt": (_I, [_P, _PD]),
     "vk_solver_step_count": (_I, [_P, _PU64]),

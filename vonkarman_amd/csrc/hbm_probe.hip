@@ -1,0 +1,89 @@
+// hbm_probe.hip -- practical HBM bandwidth ceilings of the access patterns the
+// passes use, measured on the device at hand (a measurement aid of bench.py,
+// not part of the solver): a contiguous device-to-device copy, and the tile
+// pattern of the strided passes -- blocks of T adjacent columns (T x 16 bytes
+// contiguous) at each of P positions, read then written at the same offsets.
+// Prints one JSON object per pattern.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+
+#define CK(x)                                                                                     \
+    do {                                                                                          \
+        hipError_t e_ = (x);                                                                      \
+        if (e_ != hipSuccess) {                                                                   \
+            std::fprintf(stderr, "hbm_probe: %s at line %d\n", hipGetErrorString(e_), __LINE__); \
+            std::exit(1);                                                                         \
+        }                                                                                         \
+    } while (0)
+
+struct d2 {
+    double a, b;
+};
+
+// grid-stride copy of n16 16-byte elements
+__global__ void k_copy(const d2 *__restrict__ in, d2 *__restrict__ out, long long n16) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long st = (long long)gridDim.x * blockDim.x;
+    for (; i < n16; i += st) out[i] = in[i];
+}
+
+// one block per tile of T columns; positions p = 0..P-1 of column c live at
+// p * pitch + c (16-byte elements); NT threads, E = T P / NT elements each
+template <int T, int P, int NT>
+__global__ __launch_bounds__(NT) void k_tiles(const d2 *__restrict__ in, d2 *__restrict__ out, long long pitch) {
+    constexpr int E = T * P / NT;
+    static_assert(T * P % NT == 0 && NT % T == 0, "tile must split evenly");
+    const int tid = threadIdx.x;
+    const long long col = (long long)blockIdx.x * T + tid % T;
+    d2 v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = in[(long long)(tid / T + e * (NT / T)) * pitch + col];
+#pragma unroll
+    for (int e = 0; e < E; ++e) out[(long long)(tid / T + e * (NT / T)) * pitch + col] = v[e];
+}
+
+int main() {
+    const long long n16 = 1ll << 27;  // 2 GiB per buffer
+    d2 *a = nullptr, *b = nullptr;
+    CK(hipMalloc(&a, n16 * 16));
+    CK(hipMalloc(&b, n16 * 16));
+    CK(hipMemset(a, 0, n16 * 16));
+    CK(hipMemset(b, 0, n16 * 16));
+    hipEvent_t e0, e1;
+    CK(hipEventCreate(&e0));
+    CK(hipEventCreate(&e1));
+    auto timeit = [&](const char *name, double bytes, auto launch) {
+        for (int i = 0; i < 3; ++i) launch();
+        CK(hipGetLastError());
+        CK(hipDeviceSynchronize());
+        CK(hipEventRecord(e0));
+        for (int i = 0; i < 10; ++i) launch();
+        CK(hipEventRecord(e1));
+        CK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        CK(hipEventElapsedTime(&ms, e0, e1));
+        ms /= 10.f;
+        std::printf("{\"pattern\": \"%s\", \"ms\": %.4f, \"GBps\": %.1f}\n", name, ms, bytes / (ms * 1e-3) / 1e9);
+        std::fflush(stdout);
+    };
+    timeit("contiguous copy, 1024 x 256 threads", 2.0 * n16 * 16,
+           [&] { hipLaunchKernelGGL(k_copy, dim3(1024), dim3(256), 0, 0, a, b, n16); });
+    // P = 384 positions (the padded length at 256^3); pitch = columns per
+    // position row, a multiple of 32 so every tile is whole
+    constexpr int P = 384;
+    const long long pitch = n16 / P / 32 * 32;
+    const double bytes = 2.0 * P * pitch * 16;
+    timeit("tiles of 4 columns (64 B rows)", bytes,
+           [&] { hipLaunchKernelGGL((k_tiles<4, P, 128>), dim3(pitch / 4), dim3(128), 0, 0, a, b, pitch); });
+    timeit("tiles of 8 columns (128 B rows)", bytes,
+           [&] { hipLaunchKernelGGL((k_tiles<8, P, 256>), dim3(pitch / 8), dim3(256), 0, 0, a, b, pitch); });
+    timeit("tiles of 16 columns (256 B rows)", bytes,
+           [&] { hipLaunchKernelGGL((k_tiles<16, P, 512>), dim3(pitch / 16), dim3(512), 0, 0, a, b, pitch); });
+    timeit("tiles of 32 columns (512 B rows)", bytes,
+           [&] { hipLaunchKernelGGL((k_tiles<32, P, 1024>), dim3(pitch / 32), dim3(1024), 0, 0, a, b, pitch); });
+    CK(hipFree(a));
+    CK(hipFree(b));
+    return 0;
+}

@@ -315,6 +315,11 @@ struct vk_solver {
     bool vjp_ready = false;
     DevBuf sa_, sb_, sc_, lam, gu, g1p, cot;  // stage states a, b, c and cotangents
     DevBuf WA, WB;                            // VJP work arrays
+    // physical-space spectra (u, w: the y pass output, 6 fields, plus 3 slots
+    // for the cotangent's y pass) of the stage states u, a, b, kept from the
+    // stage recompute so their VJPs skip the u-branch x and y passes
+    DevBuf KB[3];
+    bool keep_b = false;
     std::vector<DevBuf> snaps;                 // checkpointed states
     std::vector<int> split_tab;                // binomial schedule: best first advance per (steps, free slots)
     int split_T = 0, split_S = 0;
@@ -334,8 +339,10 @@ namespace {
 //   fuse:    the stage pass also runs the x inverse of its stage output into A
 //            for the next evaluation (sa.out, if set, still receives the stage
 //            output).
+//   bbuf:    the y-pass output (u, w in y-physical space, 6 fields) goes to
+//            bbuf (field stride M M Hp) instead of the work array B.
 int nl_eval(vk_solver *s, const void *src, StageArgs &sa, vk_timing *t, hipEvent_t *ev, bool skip_p1 = false,
-            bool fuse = false) {
+            bool fuse = false, void *bbuf = nullptr) {
     const int N = s->n, M = s->m, Hp = s->hp, H = s->h;
     const long long NMH = (long long)N * M * Hp, MMH = (long long)M * M * Hp, NH = (long long)N * Hp;
     const long long MH = (long long)M * Hp;
@@ -351,12 +358,13 @@ int nl_eval(vk_solver *s, const void *src, StageArgs &sa, vk_timing *t, hipEvent
     if (int r = mark(1)) return r;
     // P2: y inverse with padding, curl completed: A -> B[6][M][M][Hp]
     Strided g2 = make_strided((int)MH, Hp, H, (int)MH, 0, MH, NMH, Hp, MH, Hp, MMH);
-    VK_TRY(launch_inv(INV_UOMEGA, M, N, s->wp, s->wp, s->A.p, s->B.p, s->tw_m.p, g2, N, s->ks, 1.0, s->st));
+    void *yb = bbuf ? bbuf : s->B.p;
+    VK_TRY(launch_inv(INV_UOMEGA, M, N, s->wp, s->wp, s->A.p, yb, s->tw_m.p, g2, N, s->ks, 1.0, s->st));
     if (int r = mark(2)) return r;
     // P3: z C2R, u x w, R2C, truncate: B -> C (in A) [3][M][M][Hp]
     const double nt = (double)N * N * N;
     const double inv_scale = 1.0 / ((double)M * M * M / nt);
-    VK_TRY(launch_z_nl(M, s->wp, s->B.p, s->A.p, (long long)M * M, Hp, MMH, MMH, 1.0 / nt, inv_scale, s->tw_m.p, s->st));
+    VK_TRY(launch_z_nl(M, s->wp, yb, s->A.p, (long long)M * M, Hp, MMH, MMH, 1.0 / nt, inv_scale, s->tw_m.p, s->st));
     if (int r = mark(3)) return r;
     // P2': y forward, truncate: C (A) -> D (B) [3][N][M][Hp] (y-major)
     Strided g3 = make_strided((int)MH, Hp, H, Hp, MH, Hp, MMH, (int)MH, 0, MH, NMH);
@@ -541,6 +549,7 @@ int vk_solver_destroy(vk_solver *s) {
                       &s->cot, &s->WA, &s->WB})
         dev_free(*b);
     for (DevBuf &b : s->snaps) dev_free(b);
+    for (DevBuf &b : s->KB) dev_free(b);
     for (auto &e : s->ev)
         if (e) (void)hipEventDestroy(e);
     if (s->st) (void)hipStreamDestroy(s->st);
@@ -631,6 +640,25 @@ int vk_solver_step_timed(vk_solver *s, int nsteps, vk_timing *t) {
     t->steps += nsteps;
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
+    return VK_OK;
+}
+
+// rk4_step (vonkarman-periodic/src/rk4.rs:16-146): nsteps classical RK4 steps
+// of du/dt = N(u) - nu k^2 u on the state; the clock and dt are not touched
+// (the reference's rk4_step is a free function of u_hat).
+int vk_solver_rk4_step(vk_solver *s, double dt, int nsteps) {
+    if (int r = check(s)) return r;
+    if (nsteps < 0 || !(dt == dt)) return fail(VK_ERR_ARG, "vk_solver_rk4_step: bad dt or step count");
+    static const int stages[4] = {RK_1, RK_2, RK_3, RK_4};
+    for (int k = 0; k < nsteps; ++k)
+        for (int i = 0; i < 4; ++i) {
+            StageArgs sa = stage_args(s, stages[i], i == 3 ? s->u.p : s->x.p);
+            sa.xin = i == 0 ? s->u.p : s->x.p;
+            sa.nu = s->nu;
+            sa.rk_dt = dt;
+            if (int r = nl_eval(s, i == 0 ? s->u.p : s->x.p, sa, nullptr, nullptr)) return r;
+        }
+    VK_HIP(hipStreamSynchronize(s->st));
     return VK_OK;
 }
 
@@ -814,6 +842,14 @@ int ensure_vjp(vk_solver *s) {
         (r = dev_alloc(s->cot, cot_bytes, s->total)) || (r = dev_alloc(s->WA, wa * s->wbytes(), s->total)) ||
         (r = dev_alloc(s->WB, wb * s->wbytes(), s->total)))
         return r;
+    // the kept y-pass outputs, when they fit with 8 GB to spare (else the
+    // VJPs recompute them)
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b > 3 * wb * s->wbytes() + (size_t(8) << 30)) {
+        s->keep_b = true;
+        for (auto &b : s->KB)
+            if ((r = dev_alloc(b, wb * s->wbytes(), s->total))) return r;
+    }
     VK_HIP(hipMemsetAsync(s->lam.p, 0, cot_bytes, s->st));
     s->vjp_ready = true;
     return 0;
@@ -831,8 +867,10 @@ StageArgs adj_args(vk_solver *s, int stage, void *out) {
 }
 
 // t = N'(v)^T (cotangent prepared in s->cot as inv_scale * P(lam)), then the
-// reverse-stage epilogue sa.  v is a state-layout field.
-int vjp_nl(vk_solver *s, const void *v, StageArgs &sa) {
+// reverse-stage epilogue sa.  v is a state-layout field.  pre: a 9-field
+// buffer whose fields 0..5 already hold v's y-pass output (u, w), kept from
+// the stage recompute; the u-branch x and y passes are then skipped.
+int vjp_nl(vk_solver *s, const void *v, StageArgs &sa, void *pre = nullptr) {
     const int N = s->n, M = s->m, Hp = s->hp, H = s->h;
     const long long NMH = (long long)N * M * Hp, MMH = (long long)M * M * Hp, NH = (long long)N * Hp;
     const long long MH = (long long)M * Hp;
@@ -840,16 +878,17 @@ int vjp_nl(vk_solver *s, const void *v, StageArgs &sa) {
     const double nt = (double)N * N * N;
     // x inverse: u branch (5 fields) and cotangent branch (3 fields) -> WA
     Strided g1 = make_strided((int)NH, Hp, H, (int)NH, 0, NH, s->S, Hp, MH, Hp, NMH);
-    VK_TRY(launch_inv(INV_U3DX, M, N, s->wp, s->wp, v, s->WA.p, s->tw_m.p, g1, N, s->ks, 1.0, s->st));
+    void *wb9 = pre ? pre : s->WB.p;  // the z pass input: u, w, g
+    if (!pre) VK_TRY(launch_inv(INV_U3DX, M, N, s->wp, s->wp, v, s->WA.p, s->tw_m.p, g1, N, s->ks, 1.0, s->st));
     VK_TRY(launch_inv(INV_U3, M, N, F64, s->wp, s->cot.p, woff(s->WA.p, 5 * NMH, wb), s->tw_m.p, g1, N, s->ks, 1.0,
                       s->st));
-    // y inverse: u, w (6 fields) and g (3 fields) -> WB
+    // y inverse: u, w (6 fields) and g (3 fields) -> WB (or the kept buffer)
     Strided g2 = make_strided((int)MH, Hp, H, (int)MH, 0, MH, NMH, Hp, MH, Hp, MMH);
-    VK_TRY(launch_inv(INV_UOMEGA, M, N, s->wp, s->wp, s->WA.p, s->WB.p, s->tw_m.p, g2, N, s->ks, 1.0, s->st));
-    VK_TRY(launch_inv(INV_U3, M, N, s->wp, s->wp, woff(s->WA.p, 5 * NMH, wb), woff(s->WB.p, 6 * MMH, wb), s->tw_m.p,
-                      g2, N, s->ks, 1.0, s->st));
+    if (!pre) VK_TRY(launch_inv(INV_UOMEGA, M, N, s->wp, s->wp, s->WA.p, wb9, s->tw_m.p, g2, N, s->ks, 1.0, s->st));
+    VK_TRY(launch_inv(INV_U3, M, N, s->wp, s->wp, woff(s->WA.p, 5 * NMH, wb), woff(wb9, 6 * MMH, wb), s->tw_m.p, g2,
+                      N, s->ks, 1.0, s->st));
     // z: GU = C2R^T(w x g), GW = C2R^T(g x u) -> WA (6 fields)
-    VK_TRY(launch_z_vjp(M, s->wp, s->WB.p, s->WA.p, (long long)M * M, Hp, MMH, MMH, 1.0 / nt, 1.0 / nt, s->tw_m.p,
+    VK_TRY(launch_z_vjp(M, s->wp, wb9, s->WA.p, (long long)M * M, Hp, MMH, MMH, 1.0 / nt, 1.0 / nt, s->tw_m.p,
                         s->st));
     // y forward + truncation of the 6 fields -> WB (y-major)
     Strided g3 = make_strided((int)MH, Hp, H, Hp, MH, Hp, MMH, (int)MH, 0, MH, NMH);
@@ -867,12 +906,15 @@ int step_vjp(vk_solver *s) {
     // (with the forward step's pass fusion, so a and b are computed by the
     // same kernels as in the step; they are also stored for the reverse sweep)
     const bool fz = s->fuse_x;
+    void *kb[3] = {nullptr, nullptr, nullptr};
+    if (s->keep_b)
+        for (int i = 0; i < 3; ++i) kb[i] = s->KB[i].p;
     StageArgs f1 = stage_args(s, ST_1, s->sa_.p);
-    if (int r = nl_eval(s, s->u.p, f1, nullptr, nullptr, false, fz)) return r;
+    if (int r = nl_eval(s, s->u.p, f1, nullptr, nullptr, false, fz, kb[0])) return r;
     StageArgs f2 = stage_args(s, ST_2, s->sb_.p);
-    if (int r = nl_eval(s, s->sa_.p, f2, nullptr, nullptr, fz, fz)) return r;
+    if (int r = nl_eval(s, s->sa_.p, f2, nullptr, nullptr, fz, fz, kb[1])) return r;
     StageArgs f3 = stage_args(s, ST_3, s->sc_.p);
-    if (int r = nl_eval(s, s->sb_.p, f3, nullptr, nullptr, fz, false)) return r;
+    if (int r = nl_eval(s, s->sb_.p, f3, nullptr, nullptr, fz, false, kb[2])) return r;
     // reverse sweep
     StageArgs tmp = adj_args(s, AS_4, nullptr);
     VK_TRY(launch_adj_prep(static_cast<const double *>(s->lam.p), static_cast<double *>(s->cot.p), s->n, s->hp,
@@ -881,11 +923,11 @@ int step_vjp(vk_solver *s) {
     StageArgs a4 = adj_args(s, AS_4, nullptr);
     if (int r = vjp_nl(s, s->sc_.p, a4)) return r;
     StageArgs a3 = adj_args(s, AS_3, nullptr);
-    if (int r = vjp_nl(s, s->sb_.p, a3)) return r;
+    if (int r = vjp_nl(s, s->sb_.p, a3, kb[2])) return r;
     StageArgs a2 = adj_args(s, AS_2, nullptr);
-    if (int r = vjp_nl(s, s->sa_.p, a2)) return r;
+    if (int r = vjp_nl(s, s->sa_.p, a2, kb[1])) return r;
     StageArgs a1 = adj_args(s, AS_1, s->lam.p);
-    if (int r = vjp_nl(s, s->u.p, a1)) return r;
+    if (int r = vjp_nl(s, s->u.p, a1, kb[0])) return r;
     return 0;
 }
 

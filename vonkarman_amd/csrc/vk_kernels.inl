@@ -35,15 +35,19 @@ enum KernelClass { K_INV, K_FWD, K_FVJP, K_ZNL, K_ZVJP };
 #ifndef VK_WAVES_INV_F64
 #define VK_WAVES_INV_F64 1
 #endif
+#ifndef VK_WAVES_ZVJP_F64
+#define VK_WAVES_ZVJP_F64 1
+#endif
 #ifndef VK_WAVES_FWD_F64
 #define VK_WAVES_FWD_F64 2  // 256 VGPRs: keeps the stage-3/4 epilogues at 2 waves (no AGPR overflow)
 #endif
 template <int K, typename TW>
 struct Waves {
-    static constexpr int value = sizeof(TW) == 8 ? (K == K_ZNL   ? VK_WAVES_ZNL_F64
-                                                    : K == K_INV ? VK_WAVES_INV_F64
-                                                    : K == K_FWD ? VK_WAVES_FWD_F64
-                                                                 : 1)
+    static constexpr int value = sizeof(TW) == 8 ? (K == K_ZNL    ? VK_WAVES_ZNL_F64
+                                                    : K == K_INV  ? VK_WAVES_INV_F64
+                                                    : K == K_FWD  ? VK_WAVES_FWD_F64
+                                                    : K == K_ZVJP ? VK_WAVES_ZVJP_F64
+                                                                  : 1)
                                  : K == K_INV  ? VK_WAVES_INV_F32
                                  : K == K_FWD  ? VK_WAVES_FWD_F32
                                  : K == K_ZNL  ? VK_WAVES_ZNL_F32
@@ -251,10 +255,12 @@ __global__ __launch_bounds__((InvGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
 // 2 a21 (= a31), 3 a41, 4 b1, 5 b23, 6 b4.
 // XO: also return the stage output (the next stage input, or the new state)
 // in xo; it is stored to sa.out only when sa.out is set (uniform branch).
+// k2[i]: |k|^2 of mode i in FP64 (kx^2 + ky^2 + kz^2, as SpectralOps::k_mag_sq),
+// used by the RK4 stages' viscous term.
 template <int STG, int CH, typename TS, bool XO = false>
 __device__ __forceinline__ void stage_chunk(const StageArgs &sa, bool st, const long long (&idx)[CH],
                                             const int (&q)[CH], const cpx<TS> (&nn)[CH][3],
-                                            cpx<TS> (*xo)[3] = nullptr) {
+                                            cpx<TS> (*xo)[3] = nullptr, const double *k2 = nullptr) {
     cpx<TS> *out = reinterpret_cast<cpx<TS> *>(sa.out);
     const cpx<TS> *u = reinterpret_cast<const cpx<TS> *>(sa.u);
     cpx<TS> *n1 = reinterpret_cast<cpx<TS> *>(sa.n1);
@@ -268,6 +274,49 @@ __device__ __forceinline__ void stage_chunk(const StageArgs &sa, bool st, const 
             for (int i = 0; i < CH; ++i)
 #pragma unroll
                 for (int f = 0; f < 3; ++f) out[f * fs + idx[i]] = nn[i][f];
+        }
+    } else if constexpr (STG >= RK_1 && STG <= RK_4) {
+        // classical RK4 (vonkarman-periodic/src/rk4.rs:30-143): r = N(state) -
+        // nu k^2 state; RK_1: N1 <- r1, x <- u + dt/2 r1; RK_2: acc <- N1 +
+        // 2 r2, x <- u + dt/2 r2; RK_3: acc <- acc + 2 r3, x <- u + dt r3;
+        // RK_4: u <- u + dt/6 (acc + r4)
+        const cpx<TS> *xin = reinterpret_cast<const cpx<TS> *>(sa.xin);
+        const double dt = sa.rk_dt, nu = sa.nu;
+        cpx<TS> stv[CH][3], uu[CH][3], ac[CH][3];
+#pragma unroll
+        for (int i = 0; i < CH; ++i)
+#pragma unroll
+            for (int f = 0; f < 3; ++f) {
+                stv[i][f] = xin[f * fs + idx[i]];
+                if constexpr (STG != RK_1) uu[i][f] = u[f * fs + idx[i]];
+                if constexpr (STG == RK_2) ac[i][f] = n1[f * fs + idx[i]];
+                if constexpr (STG == RK_3 || STG == RK_4) ac[i][f] = sacc[f * fs + idx[i]];
+            }
+        if (st) {
+#pragma unroll
+            for (int i = 0; i < CH; ++i) {
+                const double visc = -nu * k2[i];
+#pragma unroll
+                for (int f = 0; f < 3; ++f) {
+                    const double rr = double(nn[i][f].re) + visc * double(stv[i][f].re);
+                    const double ri = double(nn[i][f].im) + visc * double(stv[i][f].im);
+                    const long long j = f * fs + idx[i];
+                    if constexpr (STG == RK_1) {
+                        n1[j] = {TS(rr), TS(ri)};
+                        out[j] = {TS(stv[i][f].re + 0.5 * dt * rr), TS(stv[i][f].im + 0.5 * dt * ri)};
+                    } else if constexpr (STG == RK_2) {
+                        sacc[j] = {TS(ac[i][f].re + 2.0 * rr), TS(ac[i][f].im + 2.0 * ri)};
+                        out[j] = {TS(uu[i][f].re + 0.5 * dt * rr), TS(uu[i][f].im + 0.5 * dt * ri)};
+                    } else if constexpr (STG == RK_3) {
+                        sacc[j] = {TS(ac[i][f].re + 2.0 * rr), TS(ac[i][f].im + 2.0 * ri)};
+                        out[j] = {TS(uu[i][f].re + dt * rr), TS(uu[i][f].im + dt * ri)};
+                    } else {
+                        const double sixth_dt = dt / 6.0;
+                        out[j] = {TS(uu[i][f].re + sixth_dt * (ac[i][f].re + rr)),
+                                  TS(uu[i][f].im + sixth_dt * (ac[i][f].im + ri))};
+                    }
+                }
+            }
         }
     } else if constexpr (STG == ST_1 || STG == ST_2) {
         const double dt = sa.sc->dt;
@@ -546,7 +595,7 @@ __global__ __launch_bounds__((FwdGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
     // sized so the chunk's loads fit beside the kept outputs r without
     // spilling: stages 3 and 4 read three state arrays, the reverse stages
     // read FP64 cotangents in both precisions
-    constexpr bool HEAVY = STG == ST_3 || STG == ST_4;
+    constexpr bool HEAVY = STG == ST_3 || STG == ST_4 || (STG >= RK_1 && STG <= RK_4);
     constexpr int CH = MODE == FWD_VJP ? 1 : sizeof(TW) == 8 ? (HEAVY ? 1 : 2) : (HEAVY ? 2 : 4);
     static_assert(!KEEP || K % CH == 0, "chunks must tile the kept modes");
     cpx<TW> r[KEEP ? 3 : 1][K];
@@ -577,6 +626,24 @@ __global__ __launch_bounds__((FwdGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
         wy = wavenum(iy, n);
         idx = cout + (long long)j * g.out_ls;
         q = colok ? wx * wx + wy * wy + iz * iz : 0;
+    };
+    // the same for the reverse-stage epilogue, from an opaque copy of the
+    // column: the compiler recomputes these integers instead of keeping the
+    // curl step's values live across the three GU transforms
+    auto kept2 = [&](int i, long long &idx, int &q, int &wx, int &wy, int &iz) {
+        int c = col, t = tid;
+        asm volatile("" : "+v"(c), "+v"(t));
+        const int b = i / KR, mk = i % KR;
+        const int m = mk < RP / 3 ? mk : mk + RP / 3;
+        const int p = ci_pos_out<L, RP, T, NT>(t, b * RP + m);
+        const int j = unpad_index(p, LOUT, L);
+        const int iy = c / g.hp;
+        iz = c % g.hp;
+        wx = wavenum(j, n);
+        wy = wavenum(iy, n);
+        const bool ok = c < g.ncols && iz < g.hvalid;
+        idx = (ok ? col_addr(c, g.out_inner, g.out_os) : 0) + (long long)j * g.out_ls;
+        q = ok ? wx * wx + wy * wy + iz * iz : 0;
     };
     if constexpr (MODE == FWD_VJP) {
         // t = GU + curl(GW): GW (fields 3..5) first, curled in registers, then GU added
@@ -627,7 +694,7 @@ __global__ __launch_bounds__((FwdGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
 #pragma unroll
             for (int i = 0; i < CH; ++i) {
                 int wx, wy, iz;
-                kept(c0 + i, idx[i], q[i], wx, wy, iz);
+                kept2(c0 + i, idx[i], q[i], wx, wy, iz);
                 kv[i][0] = double(wx) * ks;
                 kv[i][1] = double(wy) * ks;
                 kv[i][2] = double(iz) * ks;
@@ -659,12 +726,15 @@ __global__ __launch_bounds__((FwdGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
         for (int c0 = 0; c0 < K; c0 += CH) {
             long long idx[CH];
             int q[CH];
+            double k2d[CH];
             cpx<TW> nn[CH][3];
 #pragma unroll
             for (int i = 0; i < CH; ++i) {
                 int wx, wy, iz;
                 kept(c0 + i, idx[i], q[i], wx, wy, iz);
-                const TW kx = TW(double(wx) * ks), ky = TW(double(wy) * ks), kz = TW(double(iz) * ks);
+                const double kxd = double(wx) * ks, kyd = double(wy) * ks, kzd = double(iz) * ks;
+                k2d[i] = kxd * kxd + kyd * kyd + kzd * kzd;
+                const TW kx = TW(kxd), ky = TW(kyd), kz = TW(kzd);
                 // Leray projection (vonkarman-core/src/spectral_ops.rs:114-151)
                 cpx<TW> x = r[0][c0 + i], y = r[1][c0 + i], z = r[2][c0 + i];
                 const TW k2 = kx * kx + ky * ky + kz * kz;
@@ -680,7 +750,7 @@ __global__ __launch_bounds__((FwdGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
                 nn[i][1] = y;
                 nn[i][2] = z;
             }
-            stage_chunk<STG, CH, TW>(sa, colok, idx, q, nn);
+            stage_chunk<STG, CH, TW>(sa, colok, idx, q, nn, nullptr, k2d);
         }
     } else {
         for (int f = 0; f < NF; ++f) {
@@ -1352,6 +1422,10 @@ int run_fwd_stage(const void *in, void *out, const void *tw, const Strided &g, i
         case ST_2: return run_fwd<L, LOUT, FWD_STAGE, 3, ST_2, TW>(in, out, tw, g, n, ks, scale, sa, st);
         case ST_3: return run_fwd<L, LOUT, FWD_STAGE, 3, ST_3, TW>(in, out, tw, g, n, ks, scale, sa, st);
         case ST_4: return run_fwd<L, LOUT, FWD_STAGE, 3, ST_4, TW>(in, out, tw, g, n, ks, scale, sa, st);
+        case RK_1: return run_fwd<L, LOUT, FWD_STAGE, 3, RK_1, TW>(in, out, tw, g, n, ks, scale, sa, st);
+        case RK_2: return run_fwd<L, LOUT, FWD_STAGE, 3, RK_2, TW>(in, out, tw, g, n, ks, scale, sa, st);
+        case RK_3: return run_fwd<L, LOUT, FWD_STAGE, 3, RK_3, TW>(in, out, tw, g, n, ks, scale, sa, st);
+        case RK_4: return run_fwd<L, LOUT, FWD_STAGE, 3, RK_4, TW>(in, out, tw, g, n, ks, scale, sa, st);
     }
     return 2;
 }

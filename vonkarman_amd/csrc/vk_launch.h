@@ -57,6 +57,8 @@ enum Stage {
     ST_RAW = 0, ST_1 = 1, ST_2 = 2, ST_3 = 3, ST_4 = 4,
     // reverse sweep of the step (discrete adjoint), in the order they run
     AS_RAW = 10, AS_4 = 14, AS_3 = 13, AS_2 = 12, AS_1 = 11,
+    // classical RK4 stages (vonkarman-periodic/src/rk4.rs:16-146)
+    RK_1 = 21, RK_2 = 22, RK_3 = 23, RK_4 = 24,
 };
 struct StageArgs {
     int stage;
@@ -78,6 +80,11 @@ struct StageArgs {
     void *g1p;
     void *cot;
     double inv_scale;
+    // RK4 stages: the state the nonlinear term was evaluated at (u for RK_1,
+    // the stage input for RK_2..RK_4), the viscosity and the step
+    const void *xin;
+    double nu;
+    double rk_dt;
 };
 int launch_fwd(int mode, int L, int lout, int nf, Prec work_prec, const void *in, void *out,
                const void *tw, const Strided &g, int n, double ks, double scale,

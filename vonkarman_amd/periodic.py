@@ -187,6 +187,12 @@ class Periodic3D:
     def steps(self, nsteps: int) -> None:
         check(lib().vk_solver_step(self._h, nsteps), "vk_solver_step")
 
+    def rk4_step(self, dt: float, nsteps: int = 1) -> None:
+        """rk4_step (vonkarman-periodic/src/rk4.rs:16-146): classical RK4 on
+        du/dt = N(u) - nu |k|^2 u with step dt, applied to the state (the
+        ETD-RK4 cross-check; time, step count and dt are left unchanged)."""
+        check(lib().vk_solver_rk4_step(self._h, float(dt), int(nsteps)), "vk_solver_rk4_step")
+
     def set_fusion(self, enable: bool) -> None:
         """Fused x pass (stage epilogue + the next stage's x inverse in one
         kernel; on by default).  A performance switch of this backend with no
