@@ -33,8 +33,9 @@ def pass_bytes(n: int, w: int) -> dict:
     arrays are in the work precision).  N = n, M = 3n/2, H = n/2+1.
     One state-layout unit U = 3 N N H complex.  Stage epilogue traffic in U:
     ST_1 reads u, writes N1 and the stage input; ST_2 reads u, writes N2 and
-    the stage input; ST_3 reads u, N1, N2 and writes N2+N3 and the stage input;
-    ST_4 reads u, N1, N2+N3 and writes u.  The fused x pass (stages 1-3) does
+    the stage input; ST_3 reads u, N1, N2 and writes the N4-free part of the
+    final combine and the stage input; ST_4 reads that part and writes u.  The
+    fused x pass (stages 1-3) does
     not write the stage input and writes the next evaluation's x-inverse
     output (5 fields of M N H) instead of that evaluation's x pass reading it."""
     N, M, H = n, 3 * n // 2, n // 2 + 1
@@ -46,8 +47,8 @@ def pass_bytes(n: int, w: int) -> dict:
         "y_inverse": 5 * mnh * c + 6 * mmh * c,
         "z_cross": 6 * mmh * c + 3 * mmh * c,
         "y_forward": 3 * mmh * c + 3 * mnh * c,
-        "x_forward": 3 * mnh * c + 4 * U,                                # stage 4
-        "x_forward_unfused": 3 * mnh * c + (3 + 3 + 5 + 4) / 4.0 * U,    # average of stages 1-4
+        "x_forward": 3 * mnh * c + 2 * U,                                # stage 4
+        "x_forward_unfused": 3 * mnh * c + (3 + 3 + 5 + 2) / 4.0 * U,    # average of stages 1-4
         "x_fused": 3 * mnh * c + (2 + 2 + 4) / 3.0 * U + 5 * mnh * c,    # average of stages 1-3
         "cfl": U + 3 * nnh * c + 3 * nnh * c + 3 * nnh * c + 3 * nnh * c,
     }

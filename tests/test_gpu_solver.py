@@ -218,3 +218,47 @@ def test_checkpoint_restart_bitwise(hip_device, tmp_path):
     assert np.array_equal(a.view(np.float64), b.view(np.float64))
     assert abs(cont.time() - rest.time()) < 1e-14
     assert cont.step_count() == rest.step_count() == 100
+
+
+def test_snapshot_velocity_and_vorticity(hip_device):
+    """Domain::snapshot (solver.rs:589-623): physical u and w = curl u (curl on
+    the device) for Taylor-Green n=16, against the closed forms
+    u = (sx cy cz, -cx sy cz, 0), w = (-cx sy sz, -sx cy sz, 2 sx sy cz) and the
+    oracle's curl + c2r."""
+    from vonkarman_amd import GridSpec, IcType, Periodic3D
+
+    print("case snapshot tg16", flush=True)
+    n = 16
+    s = Periodic3D(GridSpec.cubic(n), 0.01, IcType.TaylorGreen())
+    snap = s.snapshot()
+    x = np.arange(n) * 2 * np.pi / n
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    sx, cx, sy, cy, sz, cz = np.sin(X), np.cos(X), np.sin(Y), np.cos(Y), np.sin(Z), np.cos(Z)
+    u = np.stack([sx * cy * cz, -cx * sy * cz, 0 * X])
+    w = np.stack([-cx * sy * sz, -sx * cy * sz, 2 * sx * sy * cz])
+    assert np.abs(snap.velocity - u).max() < 1e-13
+    assert np.abs(snap.vorticity - w).max() < 1e-13
+    wo = np.stack([O.c2r_3d(c) for c in O.curl(s.u_hat())])
+    assert np.abs(snap.vorticity - wo).max() < 1e-14
+
+
+@pytest.mark.parametrize("prec", ["f64", "mixed"])
+def test_fused_x_pass_bitwise_equals_standalone(hip_device, prec):
+    """Setting the state from outside between steps (a restart, an adjoint
+    snapshot) leaves the trajectory bitwise unchanged: every step starts from
+    the stored state with the same passes (restarts and adjoint snapshot
+    schedules rely on it)."""
+    from vonkarman_amd import GridSpec, IcType, Periodic3D
+
+    print(f"case fused vs standalone x pass {prec}", flush=True)
+    n = 16
+    u0 = O.leray(_random_state(n, 5) * 1e-2)
+    a = Periodic3D(GridSpec.cubic(n), 0.01, IcType.Spectral(u0), precision=prec, init_dt=False)
+    b = Periodic3D(GridSpec.cubic(n), 0.01, IcType.Spectral(u0), precision=prec, init_dt=False)
+    for s in (a, b):
+        s.set_fixed_dt(2e-3)
+    a.steps(2)
+    b.steps(1)
+    b.set_u_hat(b.u_hat())
+    b.steps(1)
+    assert np.array_equal(a.u_hat().view(np.float64), b.u_hat().view(np.float64))

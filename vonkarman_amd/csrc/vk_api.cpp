@@ -385,15 +385,15 @@ int nl_eval(vk_solver *s, const void *src, StageArgs &sa, vk_timing *t, hipEvent
 int max_pass(vk_solver *s, bool omega) {
     const int N = s->n, Hp = s->hp, H = s->h;
     const long long NH = (long long)N * Hp, NNH = (long long)N * N * Hp;
+    void *t1 = s->A.p, *t2 = s->B.p;
     VK_TRY(launch_fill_u64(&s->dsc()->umax2_bits, 0ull, 1, s->st));
     Strided g1 = make_strided((int)NH, Hp, H, (int)NH, 0, NH, s->S, Hp, NH, Hp, NNH);
-    VK_TRY(launch_inv(omega ? INV_U3DX : INV_U3, N, N, s->wp, s->wp, s->u.p, s->A.p, s->tw_n.p, g1, N, s->ks, 1.0,
+    VK_TRY(launch_inv(omega ? INV_U3DX : INV_U3, N, N, s->wp, s->wp, s->u.p, t1, s->tw_n.p, g1, N, s->ks, 1.0,
                       s->st));
     Strided g2 = make_strided((int)NH, Hp, H, (int)NH, 0, NH, NNH, Hp, NH, Hp, NNH);
-    VK_TRY(launch_inv(omega ? INV_OMEGA : INV_U3, N, N, s->wp, s->wp, s->A.p, s->B.p, s->tw_n.p, g2, N, s->ks, 1.0,
-                      s->st));
+    VK_TRY(launch_inv(omega ? INV_OMEGA : INV_U3, N, N, s->wp, s->wp, t1, t2, s->tw_n.p, g2, N, s->ks, 1.0, s->st));
     const double nt = (double)N * N * N;
-    VK_TRY(launch_z_max(N, s->wp, s->B.p, (long long)N * N, Hp, NNH, 1.0 / nt, s->tw_n.p, s->dsc(), s->st));
+    VK_TRY(launch_z_max(N, s->wp, t2, (long long)N * N, Hp, NNH, 1.0 / nt, s->tw_n.p, s->dsc(), s->st));
     return 0;
 }
 
@@ -445,7 +445,11 @@ int one_step(vk_solver *s, vk_timing *t) {
     static const int stages[4] = {ST_1, ST_2, ST_3, ST_4};
     for (int k = 0; k < 4; ++k) {
         // fused: stages 1-3 hand the next stage input to the next evaluation
-        // through its x inverse in A; the stage inputs are not stored
+        // through its x pass in A; the stage inputs are not stored.  Every
+        // step starts with the standalone x pass of u, so a trajectory is
+        // bitwise the same whether or not its state was set from outside
+        // (restarts, adjoint snapshots): the fused and standalone x passes
+        // agree only to rounding.
         const bool skip = s->fuse_x && k > 0, fuse = s->fuse_x && k < 3;
         const void *src = k == 0 ? s->u.p : s->x.p;
         void *dst = k == 3 ? s->u.p : (fuse ? nullptr : s->x.p);
@@ -781,35 +785,30 @@ int vk_solver_nonlinear(vk_solver *s, const double *uhat, double *nhat) {
 
 int vk_solver_physical(vk_solver *s, double *u_phys, double *w_phys) {
     if (int r = check(s)) return r;
-    // transform u (and curl u) to physical space through a standalone plan
+    // u (and curl u, on the device) to physical space through a standalone
+    // plan; the spectra go through the unpadded staging buffer
     vk_fft_plan *p = nullptr;
     if (int r = vk_fft_plan_create(s->n, s->n, s->n, &p)) return r;
     const size_t P = (size_t)s->n * s->n * s->n;
     const size_t H = (size_t)s->n * s->n * s->h;
-    std::vector<double> uh(3 * H * 2);
-    int r = get_state_impl(s, uh.data(), false);
-    if (!r && u_phys)
-        for (int c = 0; c < 3 && !r; ++c) r = vk_fft_c2r_3d(p, uh.data() + c * H * 2, u_phys + c * P);
+    std::vector<double> sh(3 * H * 2);
+    int r = 0;
+    auto from_staging = [&](double *out) -> int {
+        if (hipMemcpyAsync(sh.data(), s->staging.p, sh.size() * sizeof(double), hipMemcpyDeviceToHost, s->st) !=
+                hipSuccess ||
+            hipStreamSynchronize(s->st) != hipSuccess)
+            return fail(VK_ERR_HIP, "vk_solver_physical: staging copy failed");
+        for (int c = 0; c < 3; ++c)
+            if (int e = vk_fft_c2r_3d(p, sh.data() + c * H * 2, out + c * P)) return e;
+        return 0;
+    };
+    if (u_phys) {
+        r = launch_repack_p(s->u.p, s->wp, s->staging.p, F64, (long long)3 * s->n * s->n, s->hp, s->h, s->h, 1, s->st);
+        if (!r) r = from_staging(u_phys);
+    }
     if (!r && w_phys) {
-        // SpectralOps::curl on the host copy (spectral_ops.rs:81-108)
-        std::vector<double> wh(3 * H * 2);
-        const int n = s->n, h = s->h;
-        for (int ix = 0; ix < n; ++ix)
-            for (int iy = 0; iy < n; ++iy)
-                for (int iz = 0; iz < h; ++iz) {
-                    const double kx = (ix < n / 2 ? ix : ix - n) * s->ks, ky = (iy < n / 2 ? iy : iy - n) * s->ks,
-                                 kz = iz * s->ks;
-                    const size_t i = ((size_t)ix * n + iy) * h + iz;
-                    const double *ux = &uh[2 * i], *uy = &uh[2 * (H + i)], *uz = &uh[2 * (2 * H + i)];
-                    double *wx = &wh[2 * i], *wy = &wh[2 * (H + i)], *wz = &wh[2 * (2 * H + i)];
-                    wx[0] = -(ky * uz[1] - kz * uy[1]);
-                    wx[1] = ky * uz[0] - kz * uy[0];
-                    wy[0] = -(kz * ux[1] - kx * uz[1]);
-                    wy[1] = kz * ux[0] - kx * uz[0];
-                    wz[0] = -(kx * uy[1] - ky * ux[1]);
-                    wz[1] = kx * uy[0] - ky * ux[0];
-                }
-        for (int c = 0; c < 3 && !r; ++c) r = vk_fft_c2r_3d(p, wh.data() + c * H * 2, w_phys + c * P);
+        r = launch_curl(s->u.p, s->wp, s->S, s->n, s->hp, s->ks, static_cast<double *>(s->staging.p), s->st);
+        if (!r) r = from_staging(w_phys);
     }
     std::string msg = g_err;
     vk_fft_plan_destroy(p);

@@ -108,6 +108,39 @@ int launch_repack(const void *src, void *dst, long long rows, int src_pitch, int
     return launch_repack_p(src, F64, dst, F64, rows, src_pitch, dst_pitch, width, nfields, st);
 }
 
+// SpectralOps::curl (vonkarman-core/src/spectral_ops.rs:81-108) of a
+// state-layout field u ([3][n][n][hp], precision TS) into unpadded complex
+// double w ([3][n][n][n/2+1])
+template <typename TS>
+__global__ void k_curl(const cpx<TS> *__restrict__ u, long long fs, int n, int hp, double ks,
+                       cpx<double> *__restrict__ w) {
+    const int h = n / 2 + 1;
+    const long long total = (long long)n * n * h, wfs = total;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int iz = (int)(i % h);
+        const long long r = i / h;  // ix * n + iy
+        const int iy = (int)(r % n), ix = (int)(r / n);
+        const double kx = double(wavenum(ix, n)) * ks, ky = double(wavenum(iy, n)) * ks, kz = double(iz) * ks;
+        const long long j = r * hp + iz;
+        const cpx<double> ux = cvt<double>(u[j]), uy = cvt<double>(u[fs + j]), uz = cvt<double>(u[2 * fs + j]);
+        w[i] = {-(ky * uz.im - kz * uy.im), ky * uz.re - kz * uy.re};
+        w[wfs + i] = {-(kz * ux.im - kx * uz.im), kz * ux.re - kx * uz.re};
+        w[2 * wfs + i] = {-(kx * uy.im - ky * ux.im), kx * uy.re - ky * ux.re};
+    }
+}
+int launch_curl(const void *u, Prec sp, long long fs, int n, int hp, double ks, double *w, hipStream_t st) {
+    const long long total = (long long)n * n * (n / 2 + 1);
+    const unsigned nb = (unsigned)std::min<long long>((total + 255) / 256, 65536);
+    if (sp == F64)
+        hipLaunchKernelGGL(k_curl<double>, dim3(nb), dim3(256), 0, st, reinterpret_cast<const cpx<double> *>(u), fs, n,
+                           hp, ks, reinterpret_cast<cpx<double> *>(w));
+    else
+        hipLaunchKernelGGL(k_curl<float>, dim3(nb), dim3(256), 0, st, reinterpret_cast<const cpx<float> *>(u), fs, n,
+                           hp, ks, reinterpret_cast<cpx<double> *>(w));
+    return last_launch();
+}
+
 __global__ void k_fill_u64(unsigned long long *p, unsigned long long v, long long n) {
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
         p[i] = v;

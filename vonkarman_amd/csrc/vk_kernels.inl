@@ -79,6 +79,25 @@ template <int L>
 struct InvPlan<L, INV_U3> {
     using type = PlanS<L>;
 };
+// The padded x pass runs the 12-element plan in reversed radix order, the
+// order of the fused stage pass's inverse (k_fwd_stage_inv).
+template <typename PLAN>
+struct ReversedPlan {
+    using radices = typename Reverse<typename PLAN::radices>::type;
+    static constexpr int E = PLAN::E;
+};
+template <int L, bool PADDED = (L % 3 == 0)>
+struct XPassPlan {
+    using type = ReversedPlan<PlanE12<L>>;
+};
+template <int L>
+struct XPassPlan<L, false> {
+    using type = typename PlanSmall<L>::type;
+};
+template <int L>
+struct InvPlan<L, INV_U3DX> {
+    using type = typename XPassPlan<L>::type;
+};
 // bytes of each position row of a strided tile (T = row bytes / element bytes columns)
 #ifndef VK_ROW_BYTES_INV
 #define VK_ROW_BYTES_INV 128
@@ -348,13 +367,17 @@ __device__ __forceinline__ void stage_chunk(const StageArgs &sa, bool st, const 
                 if (st && (!XO || out)) out[f * fs + idx[i]] = y;
             }
     } else if constexpr (STG == ST_3) {
+        // stage c, and the part of the final combine that does not involve N4:
+        // sacc <- e u + dt (b1 N1 + b23 (N2 + N3)), so stage 4 reads one array
         const double dt = sa.sc->dt;
-        double e[CH], a[CH];
+        double e[CH], a[CH], b1[CH], b23[CH];
         cpx<TS> uu[CH][3], s2[CH][3], m1[CH][3];
 #pragma unroll
         for (int i = 0; i < CH; ++i) {
             e[i] = tab[0 * ts + q[i]];
             a[i] = tab[3 * ts + q[i]];
+            b1[i] = tab[4 * ts + q[i]];
+            b23[i] = tab[5 * ts + q[i]];
 #pragma unroll
             for (int f = 0; f < 3; ++f) {
                 uu[i][f] = u[f * fs + idx[i]];
@@ -368,7 +391,10 @@ __device__ __forceinline__ void stage_chunk(const StageArgs &sa, bool st, const 
 #pragma unroll
                 for (int f = 0; f < 3; ++f) {
                     const cpx<TS> x = nn[i][f];
-                    sacc[f * fs + idx[i]] = {s2[i][f].re + x.re, s2[i][f].im + x.im};
+                    const cpx<TS> s3 = {s2[i][f].re + x.re, s2[i][f].im + x.im};  // N2 + N3
+                    const double rr = b1[i] * m1[i][f].re + b23[i] * s3.re;
+                    const double ri = b1[i] * m1[i][f].im + b23[i] * s3.im;
+                    sacc[f * fs + idx[i]] = {TS(e[i] * uu[i][f].re + dt * rr), TS(e[i] * uu[i][f].im + dt * ri)};
                 }
         }
 #pragma unroll
@@ -381,32 +407,24 @@ __device__ __forceinline__ void stage_chunk(const StageArgs &sa, bool st, const 
                 if constexpr (XO) xo[i][f] = y;
                 if (st && (!XO || out)) out[f * fs + idx[i]] = y;
             }
-    } else {  // ST_4: final combine into u
+    } else {  // ST_4: final combine into u, from the stage-3 partial in sacc
         static_assert(STG == ST_4, "unknown stage");
         const double dt = sa.sc->dt;
-        double e[CH], b1[CH], b23[CH], b4[CH];
-        cpx<TS> uu[CH][3], m1[CH][3], s[CH][3];
+        double b4[CH];
+        cpx<TS> p[CH][3];
 #pragma unroll
         for (int i = 0; i < CH; ++i) {
-            e[i] = tab[0 * ts + q[i]];
-            b1[i] = tab[4 * ts + q[i]];
-            b23[i] = tab[5 * ts + q[i]];
             b4[i] = tab[6 * ts + q[i]];
 #pragma unroll
-            for (int f = 0; f < 3; ++f) {
-                uu[i][f] = u[f * fs + idx[i]];
-                m1[i][f] = n1[f * fs + idx[i]];
-                s[i][f] = sacc[f * fs + idx[i]];
-            }
+            for (int f = 0; f < 3; ++f) p[i][f] = sacc[f * fs + idx[i]];
         }
 #pragma unroll
         for (int i = 0; i < CH; ++i)
 #pragma unroll
             for (int f = 0; f < 3; ++f) {
                 const cpx<TS> x = nn[i][f];
-                const double rr = b1[i] * m1[i][f].re + b23[i] * s[i][f].re + b4[i] * x.re;
-                const double ri = b1[i] * m1[i][f].im + b23[i] * s[i][f].im + b4[i] * x.im;
-                const cpx<TS> y = {TS(e[i] * uu[i][f].re + dt * rr), TS(e[i] * uu[i][f].im + dt * ri)};
+                const cpx<TS> y = {TS(double(p[i][f].re) + dt * (b4[i] * x.re)),
+                                   TS(double(p[i][f].im) + dt * (b4[i] * x.im))};
                 if constexpr (XO) xo[i][f] = y;
                 if (st && (!XO || out)) out[f * fs + idx[i]] = y;
             }
@@ -595,7 +613,7 @@ __global__ __launch_bounds__((FwdGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
     // sized so the chunk's loads fit beside the kept outputs r without
     // spilling: stages 3 and 4 read three state arrays, the reverse stages
     // read FP64 cotangents in both precisions
-    constexpr bool HEAVY = STG == ST_3 || STG == ST_4 || (STG >= RK_1 && STG <= RK_4);
+    constexpr bool HEAVY = STG == ST_3 || (STG >= RK_1 && STG <= RK_4);
     constexpr int CH = MODE == FWD_VJP ? 1 : sizeof(TW) == 8 ? (HEAVY ? 1 : 2) : (HEAVY ? 2 : 4);
     static_assert(!KEEP || K % CH == 0, "chunks must tile the kept modes");
     cpx<TW> r[KEEP ? 3 : 1][K];
@@ -795,7 +813,7 @@ __global__ __launch_bounds__((FwdGeom<L, FWD_STAGE, TW>::NT)) __attribute__((amd
     constexpr int RP = PL::last;
     constexpr int KR = 2 * RP / 3, K = (E / RP) * KR;
     static_assert(RP % 3 == 0 && 3 * LOUT == 2 * L, "stage pass needs a 3/2-padded plan");
-    constexpr bool HEAVY = STG == ST_3 || STG == ST_4;
+    constexpr bool HEAVY = STG == ST_3;
     constexpr int CH = sizeof(TW) == 8 ? (HEAVY ? 1 : 2) : (HEAVY ? 2 : 4);
     static_assert(K % CH == 0, "chunks must tile the kept modes");
     __shared__ cpx<TW> lds[L * T];
@@ -1011,6 +1029,17 @@ __device__ __forceinline__ void z_pair_fill(cpx<TW> (&v)[E], gptr<cpx<TW>> fa, g
 #ifndef VK_Z_ROW_BYTES_F32
 #define VK_Z_ROW_BYTES_F32 16  // T = 2 FP32 lines per block
 #endif
+// z passes read their twiddles from a per-block LDS copy: a global twiddle
+// load inside a radix pass is a dependent memory wait that a one-wave block
+// cannot hide
+#ifndef VK_Z_LDS_TW
+#define VK_Z_LDS_TW 1
+#endif
+template <int L, int D, int T, int NT, typename PL, int E, typename R>
+__device__ __forceinline__ void zfft(cpx<R> (&v)[E], cpx<R> *lds, const cpx<R> *ltw, const cpx<R> *tw, int tid) {
+    if constexpr (VK_Z_LDS_TW) fft_reg_lt<L, D, T, NT, T, 1, PL>(v, lds, ltw, tid);
+    else fft_reg<L, D, T, NT, T, 1, PL>(v, lds, tw, tid);
+}
 template <int L, typename TW>
 using ZNLP = PlanE12<L>;
 template <int L, typename TW>
@@ -1028,8 +1057,10 @@ __global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) __attribute__((amdgpu_waves_p
     constexpr int T = ZNLGeom<L, TW>::T, NT = ZNLGeom<L, TW>::NT;
     constexpr int HIN = L / 3 + 1;  // N/2+1 with N = 2L/3
     __shared__ cpx<TW> lds[L * T];
+    __shared__ cpx<TW> ltw[VK_Z_LDS_TW ? L : 1];
     const int tid = threadIdx.x;
     const long long line0 = (long long)blockIdx.x * T;
+    if constexpr (VK_Z_LDS_TW) stage_twiddles<L, NT>(ltw, tw, tid);
     const int t = tid % T;
     const bool ok = line0 + t < lines;
     // lines past the end load the last line (valid memory); their results are not stored
@@ -1042,7 +1073,7 @@ __global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) __attribute__((amdgpu_waves_p
     for (int q = 0; q < 3; ++q) {
         const gptr<cpx<TW>> fa = opaque(in) + (2 * q) * fs_in, fb = opaque(in) + (2 * q + 1) * fs_in;
         z_pair_fill<L, HIN, PL::first, T, NT, true>(v, fa, fb, base, tid);
-        fft_reg<L, +1, T, NT, T, 1, PL>(v, lds, tw, tid);
+        zfft<L, +1, T, NT, PL>(v, lds, ltw, tw, tid);
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             r[2 * q][e] = v[e].re * s_in;
@@ -1063,7 +1094,7 @@ __global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) __attribute__((amdgpu_waves_p
     // (c0 + i c1): forward, then split through LDS (needs k and L-k)
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = {r[0][e], r[1][e]};
-    fft_reg<L, -1, T, NT, T, 1, RL>(v, lds, tw, tid);
+    zfft<L, -1, T, NT, RL>(v, lds, ltw, tw, tid);
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < E; ++e) lds[ci_pos_out<L, RL::last, T, NT>(tid, e) * T + t] = v[e];
@@ -1082,7 +1113,7 @@ __global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) __attribute__((amdgpu_waves_p
     // c2 alone: its first HOUT outputs are stored from registers
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = {r[2][e], TW(0)};
-    fft_reg<L, -1, T, NT, T, 1, RL>(v, lds, tw, tid);
+    zfft<L, -1, T, NT, RL>(v, lds, ltw, tw, tid);
     if (ok) {
 #pragma unroll
         for (int e = 0; e < E; ++e) {
@@ -1131,8 +1162,10 @@ __global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) __attribute__((amdgpu_waves_p
     constexpr int T = ZNLGeom<L, TW>::T, NT = ZNLGeom<L, TW>::NT;
     constexpr int HIN = L / 3 + 1, HOUT = HIN;
     __shared__ cpx<TW> lds[L * T];
+    __shared__ cpx<TW> ltw[VK_Z_LDS_TW ? L : 1];
     const int tid = threadIdx.x;
     const long long line0 = (long long)blockIdx.x * T;
+    if constexpr (VK_Z_LDS_TW) stage_twiddles<L, NT>(ltw, tw, tid);
     const int t = tid % T;
     const bool ok = line0 + t < lines;
     // lines past the end load the last line (valid memory); their results are not stored
@@ -1150,7 +1183,7 @@ __global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) __attribute__((amdgpu_waves_p
         else z_pair_fill<L, HIN, PL::first, T, NT, false>(v, pa, pa, base, tid);
         // position 0 is element 0 of thread column 0's first row
         if (ci_pos_in<L, PL::first, T, NT>(tid, 0) == 0) v[0] = {v[0].re * dca, v[0].im * dcb};
-        fft_reg<L, +1, T, NT, T, 1, PL>(v, lds, tw, tid);
+        zfft<L, +1, T, NT, PL>(v, lds, ltw, tw, tid);
     };
     // Cotangent fields are paired only with cotangent fields: a complex FFT
     // of (a + i b) leaks roundoff of the larger of a, b into the other, and u, w
@@ -1186,7 +1219,7 @@ __global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) __attribute__((amdgpu_waves_p
         v[e] = {gw0, gw1};
     }
     __syncthreads();
-    fft_reg<L, -1, T, NT, T, 1, RL>(v, lds, tw, tid);
+    zfft<L, -1, T, NT, RL>(v, lds, ltw, tw, tid);
     __syncthreads();
     z_split_store<L, T, NT, RL, E, HOUT>(v, lds, tid, line0, lines, hp, out + 3 * fs_out, out + 4 * fs_out, s_o);
     // live: g0, g1, g2 = r0, gU1 part = r1, gU2 part = r2, gW2 = r3
@@ -1200,12 +1233,12 @@ __global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) __attribute__((amdgpu_waves_p
         v[e] = {gu0, gu1};
     }
     __syncthreads();
-    fft_reg<L, -1, T, NT, T, 1, RL>(v, lds, tw, tid);
+    zfft<L, -1, T, NT, RL>(v, lds, ltw, tw, tid);
     __syncthreads();
     z_split_store<L, T, NT, RL, E, HOUT>(v, lds, tid, line0, lines, hp, out + 0 * fs_out, out + 1 * fs_out, s_o);
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = {r0[e], r3[e]};
-    fft_reg<L, -1, T, NT, T, 1, RL>(v, lds, tw, tid);
+    zfft<L, -1, T, NT, RL>(v, lds, ltw, tw, tid);
     __syncthreads();
     z_split_store<L, T, NT, RL, E, HOUT>(v, lds, tid, line0, lines, hp, out + 2 * fs_out, out + 5 * fs_out, s_o);
 }

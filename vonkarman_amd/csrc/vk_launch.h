@@ -67,7 +67,8 @@ struct StageArgs {
     void *out;        // ST_RAW: N hat; ST_1..3: next stage input; ST_4: u (in place)
     const void *u;    // state
     void *n1;         // N1 (written by ST_1, read by ST_3/ST_4)
-    void *sacc;       // N2+N3 (written by ST_2/ST_3, read by ST_4)
+    void *sacc;       // N2 (ST_2), then e u + dt (b1 N1 + b23 (N2+N3)) (ST_3), read by ST_4;
+                      // RK4: k1 + 2 k2 (+ 2 k3)
     long long fs;     // field stride of the state-layout arrays
     const double *tab;  // ETD table, 7 arrays of (qmax+1) doubles
     int tab_stride;     // qmax+1
@@ -128,6 +129,8 @@ int launch_repack(const void *src, void *dst, long long rows, int src_pitch, int
 int launch_repack_p(const void *src, Prec sprec, void *dst, Prec dprec, long long rows, int src_pitch, int dst_pitch,
                     int width, int nfields, hipStream_t st);
 int launch_fill_u64(unsigned long long *p, unsigned long long v, long long n, hipStream_t st);
+// w = curl u: state layout (precision sp, field stride fs) -> unpadded complex double [3][n][n][n/2+1]
+int launch_curl(const void *u, Prec sp, long long fs, int n, int hp, double ks, double *w, hipStream_t st);
 int launch_cfl_update(Scalars *sc, double dx, double nu, double safety, int unconditional,
                       hipStream_t st);
 int launch_clock_advance(Scalars *sc, hipStream_t st);
