@@ -105,8 +105,14 @@ struct InvPlan<L, INV_U3DX> {
 #ifndef VK_ROW_BYTES_FWD
 #define VK_ROW_BYTES_FWD 128
 #endif
+#ifndef VK_ROW_BYTES_INV_F32
+#define VK_ROW_BYTES_INV_F32 VK_ROW_BYTES_INV
+#endif
+#ifndef VK_ROW_BYTES_FWD_F32
+#define VK_ROW_BYTES_FWD_F32 VK_ROW_BYTES_FWD
+#endif
 template <int L, int MODE, typename TW>
-using InvGeom = Geom<L, InvPlan<L, MODE>::type::E, TW, VK_ROW_BYTES_INV>;
+using InvGeom = Geom<L, InvPlan<L, MODE>::type::E, TW, sizeof(TW) == 8 ? VK_ROW_BYTES_INV : VK_ROW_BYTES_INV_F32>;
 
 template <int L, int LIN, int MODE, typename TI, typename TW>
 __global__ __launch_bounds__((InvGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_waves_per_eu(Waves<K_INV, TW>::value))) void k_inv(const cpx<TI> *__restrict__ in,
@@ -582,7 +588,7 @@ struct FwdPlan<L, FWD_VJP> {
     using type = PlanE12<L>;
 };
 template <int L, int MODE, typename TW>
-using FwdGeom = Geom<L, FwdPlan<L, MODE>::type::E, TW, VK_ROW_BYTES_FWD>;
+using FwdGeom = Geom<L, FwdPlan<L, MODE>::type::E, TW, sizeof(TW) == 8 ? VK_ROW_BYTES_FWD : VK_ROW_BYTES_FWD_F32>;
 
 template <int L, int LOUT, int MODE, int NF, int STG, typename TW>
 __global__ __launch_bounds__((FwdGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_waves_per_eu(Waves<MODE == FWD_VJP ? K_FVJP : K_FWD, TW>::value))) void k_fwd(const cpx<TW> *__restrict__ in,
@@ -808,8 +814,6 @@ __global__ __launch_bounds__((FwdGeom<L, FWD_STAGE, TW>::NT)) __attribute__((amd
     constexpr int E = P::E;
     using G = FwdGeom<L, FWD_STAGE, TW>;
     constexpr int T = G::T, NT = G::NT;
-    static_assert(T == InvGeom<L, INV_U3DX, TW>::T && NT == InvGeom<L, INV_U3DX, TW>::NT,
-                  "the fused x passes share one tile");
     constexpr int RP = PL::last;
     constexpr int KR = 2 * RP / 3, K = (E / RP) * KR;
     static_assert(RP % 3 == 0 && 3 * LOUT == 2 * L, "stage pass needs a 3/2-padded plan");
