@@ -194,7 +194,7 @@ int vk_fft_plan_create(int nx, int ny, int nz, vk_fft_plan **out) {
     p->ny = ny;
     p->nz = nz;
     p->hz = nz / 2 + 1;
-    p->hp = round_up(p->hz, 8);
+    p->hp = round_up(p->hz, VK_HP_ALIGN_BYTES / 16);
     p->name = "vonkarman-hip (gfx950 Stockham, fp64)";
     int r = 0;
     const size_t ncplx = (size_t)nx * ny * p->hp;
@@ -490,9 +490,6 @@ int vk_solver_create(int n, double l, double nu, int precision, vk_solver **out)
     s->m = 3 * n / 2;  // GridSpec::padded_3half (vonkarman-core/src/field.rs:53-62)
     s->h = n / 2 + 1;
     s->wp = precision == VK_PREC_F64 ? F64 : F32;
-#ifndef VK_HP_ALIGN_BYTES
-#define VK_HP_ALIGN_BYTES 128
-#endif
     // spectral rows padded to whole tile rows (VK_HP_ALIGN_BYTES per row)
     const int row_elems = VK_HP_ALIGN_BYTES / (s->wp == F64 ? 16 : 8);
     s->hp = round_up(s->h, row_elems);

@@ -58,6 +58,24 @@ __device__ __forceinline__ long long col_addr(int col, int inner, long long os) 
     return (long long)(col / inner) * os + (col % inner);
 }
 
+// Output transforms of the strided inverse passes are stored by every column
+// of the tile, padding columns ((col % hp) >= hvalid) included: their values
+// land in row padding, which no pass reads.  This needs whole tiles
+// (ncols % T == 0), which holds when T divides the row alignment (every
+// strided geometry spans whole padded rows); tiles widened for short lines
+// keep predicated stores, and the launchers check.  A store under a
+// per-column predicate ends in a branch join where the compiler merges its
+// outstanding-memory counts conservatively, and the next field's prefetched
+// loads are then waited for together with every store of this field (loads
+// and stores share one in-order counter).
+#ifndef VK_STORE_PAD
+#define VK_STORE_PAD 1
+#endif
+template <typename TW, int T>
+constexpr bool pad_stores() {
+    return VK_STORE_PAD && (VK_HP_ALIGN_BYTES / (2 * (int)sizeof(TW))) % T == 0;
+}
+
 // ===========================================================================
 // Inverse strided pass: lines of LIN spectral inputs zero-padded to L, inverse
 // DFT, all L physical outputs stored.  Curl terms are formed on the way in:
@@ -138,7 +156,8 @@ __global__ __launch_bounds__((InvGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
     const int col = col0 + tid % T;
     const bool colok = col < g.ncols && (col % g.hp) < g.hvalid;
     const long long cin = colok ? col_addr(col, g.in_inner, g.in_os) : 0;
-    const long long cout = colok ? col_addr(col, g.out_inner, g.out_os) : 0;
+    constexpr bool PADST = pad_stores<TW, T>();
+    const long long cout = (PADST || colok) ? col_addr(col, g.out_inner, g.out_os) : 0;
     const TW kz_col = TW(double(col % g.hp) * ks);  // kz of the column (y pass)
     // element e -> unpadded input index (-1 in the zero gap)
     auto uidx = [&](int e) -> int {
@@ -185,7 +204,7 @@ __global__ __launch_bounds__((InvGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
         for (int e = 0; e < E; ++e) v[e] = value(e);
         if constexpr (PF) fft_reg_lt<L, +1, T, NT, T, 1, PL>(v, lds, ltw, tid);
         else fft_reg<L, +1, T, NT, T, 1, PL>(v, lds, tw, tid);
-        if (colok) {
+        if (PADST || colok) {
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 const int p = ci_pos_out<L, PL::last, T, NT>(tid, e);
@@ -832,7 +851,8 @@ __global__ __launch_bounds__((FwdGeom<L, FWD_STAGE, TW>::NT)) __attribute__((amd
     const bool colok = col < g.ncols && (col % g.hp) < g.hvalid;
     const long long cin = colok ? col_addr(col, g.in_inner, g.in_os) : 0;
     const long long cout = colok ? col_addr(col, g.out_inner, g.out_os) : 0;
-    const long long cinv = colok ? col_addr(col, gi.out_inner, gi.out_os) : 0;
+    constexpr bool PADST = pad_stores<TW, T>();
+    const long long cinv = (PADST || colok) ? col_addr(col, gi.out_inner, gi.out_os) : 0;
     auto load_into = [&](cpx<TW>(&S)[E], int f) {
         const gptr<cpx<TW>> src = opaque(in) + f * g.in_fs + cin;
 #pragma unroll
@@ -907,7 +927,7 @@ __global__ __launch_bounds__((FwdGeom<L, FWD_STAGE, TW>::NT)) __attribute__((amd
 #pragma unroll
         for (int e = 0; e < E; ++e) v[e] = value(e);
         fft_reg_lt<L, +1, T, NT, T, 1, RL>(v, lds, ltw, tid);
-        if (colok) {
+        if (PADST || colok) {
 #pragma unroll
             for (int e = 0; e < E; ++e)
                 aout[fo * gi.out_fs + cinv + (long long)ci_pos_out<L, RL::last, T, NT>(tid, e) * gi.out_ls] = v[e];
@@ -925,6 +945,7 @@ template <int L, int LOUT, int STG, typename TW>
 int run_fwd_inv(const void *in, void *aout, const void *tw, const Strided &g, const Strided &gi, int n, double ks,
                 const StageArgs *sa, hipStream_t st) {
     using C = FwdGeom<L, FWD_STAGE, TW>;
+    if (pad_stores<TW, C::T>() && g.ncols % C::T != 0) return 2;
     const unsigned nb = (unsigned)((g.ncols + C::T - 1) / C::T);
     hipLaunchKernelGGL((k_fwd_stage_inv<L, LOUT, STG, TW>), dim3(nb), dim3(C::NT), 0, st,
                        reinterpret_cast<const cpx<TW> *>(in), reinterpret_cast<cpx<TW> *>(aout),
@@ -1384,6 +1405,7 @@ template <int L, int LIN, int MODE, typename TI, typename TW>
 int run_inv(const void *in, void *out, const void *tw, const Strided &g, int n, double ks, double scale,
             hipStream_t st) {
     using C = InvGeom<L, MODE, TW>;
+    if (pad_stores<TW, C::T>() && g.ncols % C::T != 0) return 2;
     const unsigned nb = (unsigned)((g.ncols + C::T - 1) / C::T);
     hipLaunchKernelGGL((k_inv<L, LIN, MODE, TI, TW>), dim3(nb), dim3(C::NT), 0, st,
                        reinterpret_cast<const cpx<TI> *>(in), reinterpret_cast<cpx<TW> *>(out),

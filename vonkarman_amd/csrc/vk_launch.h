@@ -4,6 +4,12 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// Spectral rows (the kz axis, h = n/2 + 1 modes) are padded to hp, a multiple
+// of VK_HP_ALIGN_BYTES / element bytes, so strided tiles cover whole rows.
+#ifndef VK_HP_ALIGN_BYTES
+#define VK_HP_ALIGN_BYTES 128
+#endif
+
 namespace vk {
 
 enum Prec { F64 = 0, F32 = 1 };
