@@ -524,11 +524,23 @@ __device__ __forceinline__ void fft_reg_lt(cpx<R> (&v)[E], cpx<R> *lds, const cp
 // copy the twiddle table of length L into LDS; the caller synchronises
 // (the first transform's barrier comes after its first pass, which reads no
 // twiddles, so a __syncthreads() before the first transform is enough)
+// All loads are issued before the first LDS write (a load feeding a write in
+// the same loop trip is waited for before the next trip's load is issued).
 template <int L, int NT, typename R>
 __device__ __forceinline__ void stage_twiddles(cpx<R> *ltw, const cpx<R> *__restrict__ tw, int tid) {
     const gptr<cpx<R>> g = (gptr<cpx<R>>)tw;
+    constexpr int C = (L + NT - 1) / NT;
+    cpx<R> t[C];
 #pragma unroll
-    for (int i = tid; i < L; i += NT) ltw[i] = ldg(g, i);
+    for (int c = 0; c < C; ++c) {
+        const int i = tid + c * NT;
+        t[c] = ldg(g, i < L ? i : L - 1);
+    }
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const int i = tid + c * NT;
+        if (L % NT == 0 || i < L) ltw[i] = t[c];
+    }
 }
 #endif
 

@@ -1004,41 +1004,68 @@ constexpr int zpair_class(int b, int w) {
     if (b >= HIN && e - 1 <= L - HIN) return ZP_ZERO;
     return ZP_MIXED;
 }
+// One element of the pair spectrum in two steps: the raw loads (X from fa, Y
+// from fb; nothing for the zero class) and their combination into Z, so the
+// next pair's loads can be issued before the current pair's transform.
+template <typename TW>
+struct ZRaw {
+    cpx<TW> x, y;
+};
 template <int L, int HIN, bool HASB, int CLS, typename TW>
-__device__ __forceinline__ cpx<TW> z_pair_ld(gptr<cpx<TW>> fa, gptr<cpx<TW>> fb, long long base, int p) {
+__device__ __forceinline__ void z_pair_raw(ZRaw<TW> &r, gptr<cpx<TW>> fa, gptr<cpx<TW>> fb, long long base, int p) {
+    if constexpr (CLS != ZP_ZERO) {
+        const int k = CLS == ZP_DIRECT ? p : CLS == ZP_MIRROR ? L - p : (p < HIN ? p : (p > L - HIN ? L - p : 0));
+        r.x = ldg(fa, base + k);
+        if constexpr (HASB) r.y = ldg(fb, base + k);
+    }
+}
+template <int L, int HIN, bool HASB, int CLS, typename TW>
+__device__ __forceinline__ cpx<TW> z_pair_make(const ZRaw<TW> &r, int p) {
     const cpx<TW> zero = {TW(0), TW(0)};
     if constexpr (CLS == ZP_ZERO) {
         return zero;
-    } else if constexpr (CLS == ZP_DIRECT) {
-        const cpx<TW> X = ldg(fa, base + p);
-        const cpx<TW> Y = HASB ? ldg(fb, base + p) : zero;
-        return {X.re - Y.im, X.im + Y.re};
-    } else if constexpr (CLS == ZP_MIRROR) {
-        const cpx<TW> X = ldg(fa, base + (L - p));
-        const cpx<TW> Y = HASB ? ldg(fb, base + (L - p)) : zero;
-        return {X.re + Y.im, Y.re - X.im};
     } else {
-        const bool lo = p < HIN, hi = !lo && p > L - HIN, edge = p == 0 || 2 * p == L;
-        const int k = lo ? p : (hi ? L - p : 0);
-        const cpx<TW> X = ldg(fa, base + k);
-        const cpx<TW> Y = HASB ? ldg(fb, base + k) : zero;
-        const TW xi = edge ? TW(0) : X.im, yi = edge ? TW(0) : Y.im;
-        const TW zr = lo ? X.re - yi : (hi ? X.re + yi : TW(0));
-        const TW zi = lo ? xi + Y.re : (hi ? Y.re - xi : TW(0));
-        return {zr, zi};
+        const cpx<TW> X = r.x;
+        const cpx<TW> Y = HASB ? r.y : zero;
+        if constexpr (CLS == ZP_DIRECT) {
+            return {X.re - Y.im, X.im + Y.re};
+        } else if constexpr (CLS == ZP_MIRROR) {
+            return {X.re + Y.im, Y.re - X.im};
+        } else {
+            const bool lo = p < HIN, hi = !lo && p > L - HIN, edge = p == 0 || 2 * p == L;
+            const TW xi = edge ? TW(0) : X.im, yi = edge ? TW(0) : Y.im;
+            const TW zr = lo ? X.re - yi : (hi ? X.re + yi : TW(0));
+            const TW zi = lo ? xi + Y.re : (hi ? Y.re - xi : TW(0));
+            return {zr, zi};
+        }
     }
 }
-// fill the registers of a column-interleaved z tile (T lines, NT threads,
-// first radix R1) with the pair spectrum of line `base` (element e at
-// position ci_pos_in(tid, e))
+// the raw loads of a column-interleaved z tile (T lines, NT threads, first
+// radix R1) for the pair spectrum of line `base` (element e at position
+// ci_pos_in(tid, e)), and their combination into the tile's registers
 template <int L, int HIN, int R1, int T, int NT, bool HASB, int E, typename TW>
-__device__ __forceinline__ void z_pair_fill(cpx<TW> (&v)[E], gptr<cpx<TW>> fa, gptr<cpx<TW>> fb, long long base,
-                                            int tid) {
+__device__ __forceinline__ void z_pair_issue(ZRaw<TW> (&r)[E], gptr<cpx<TW>> fa, gptr<cpx<TW>> fb, long long base,
+                                             int tid) {
     static_for<E>([&](auto ec) {
         constexpr int e = decltype(ec)::value;
         constexpr int cls = zpair_class<L, HIN>(ci_in_base<L, R1, T, NT>(e), NT / T);
-        v[e] = z_pair_ld<L, HIN, HASB, cls, TW>(fa, fb, base, ci_pos_in<L, R1, T, NT>(tid, e));
+        z_pair_raw<L, HIN, HASB, cls, TW>(r[e], fa, fb, base, ci_pos_in<L, R1, T, NT>(tid, e));
     });
+}
+template <int L, int HIN, int R1, int T, int NT, bool HASB, int E, typename TW>
+__device__ __forceinline__ void z_pair_combine(cpx<TW> (&v)[E], const ZRaw<TW> (&r)[E], int tid) {
+    static_for<E>([&](auto ec) {
+        constexpr int e = decltype(ec)::value;
+        constexpr int cls = zpair_class<L, HIN>(ci_in_base<L, R1, T, NT>(e), NT / T);
+        v[e] = z_pair_make<L, HIN, HASB, cls, TW>(r[e], ci_pos_in<L, R1, T, NT>(tid, e));
+    });
+}
+template <int L, int HIN, int R1, int T, int NT, bool HASB, int E, typename TW>
+__device__ __forceinline__ void z_pair_fill(cpx<TW> (&v)[E], gptr<cpx<TW>> fa, gptr<cpx<TW>> fb, long long base,
+                                            int tid) {
+    ZRaw<TW> r[E];
+    z_pair_issue<L, HIN, R1, T, NT, HASB>(r, fa, fb, base, tid);
+    z_pair_combine<L, HIN, R1, T, NT, HASB>(v, r, tid);
 }
 
 // z tiles are column-interleaved too: the T lines of a block are the columns,
@@ -1059,6 +1086,15 @@ __device__ __forceinline__ void z_pair_fill(cpx<TW> (&v)[E], gptr<cpx<TW>> fa, g
 // cannot hide
 #ifndef VK_Z_LDS_TW
 #define VK_Z_LDS_TW 1
+#endif
+// the nonlinear z pass issues each next input pair's loads before the current
+// pair's transform (FP64: 0.738 -> 0.704 ms at 384).  Not in FP32, where the
+// raw loads push the kernel past its 128-VGPR cap (0.387 -> 0.398 ms).
+#ifndef VK_Z_PIPE_F64
+#define VK_Z_PIPE_F64 1
+#endif
+#ifndef VK_Z_PIPE_F32
+#define VK_Z_PIPE_F32 0
 #endif
 template <int L, int D, int T, int NT, typename PL, int E, typename R>
 __device__ __forceinline__ void zfft(cpx<R> (&v)[E], cpx<R> *lds, const cpx<R> *ltw, const cpx<R> *tw, int tid) {
@@ -1093,11 +1129,24 @@ __global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) __attribute__((amdgpu_waves_p
     cpx<TW> v[E];
     TW r[6][E];
 
-    // fields: 0 ux, 1 uy, 2 uz, 3 wx, 4 wy, 5 wz; pairs (0,1) (2,3) (4,5)
+    // fields: 0 ux, 1 uy, 2 uz, 3 wx, 4 wy, 5 wz; pairs (0,1) (2,3) (4,5).
+    // Each next pair's loads are in flight during the current pair's transform.
+    constexpr bool PIPE = sizeof(TW) == 8 ? VK_Z_PIPE_F64 : VK_Z_PIPE_F32;
+    ZRaw<TW> raw[E];
+    auto issue = [&](int q) {
+        z_pair_issue<L, HIN, PL::first, T, NT, true>(raw, opaque(in) + (2 * q) * fs_in,
+                                                      opaque(in) + (2 * q + 1) * fs_in, base, tid);
+    };
+    if (PIPE) issue(0);
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-        const gptr<cpx<TW>> fa = opaque(in) + (2 * q) * fs_in, fb = opaque(in) + (2 * q + 1) * fs_in;
-        z_pair_fill<L, HIN, PL::first, T, NT, true>(v, fa, fb, base, tid);
+        if (PIPE) {
+            z_pair_combine<L, HIN, PL::first, T, NT, true>(v, raw, tid);
+            if (q + 1 < 3) issue(q + 1);
+        } else {
+            const gptr<cpx<TW>> fa = opaque(in) + (2 * q) * fs_in, fb = opaque(in) + (2 * q + 1) * fs_in;
+            z_pair_fill<L, HIN, PL::first, T, NT, true>(v, fa, fb, base, tid);
+        }
         zfft<L, +1, T, NT, PL>(v, lds, ltw, tw, tid);
 #pragma unroll
         for (int e = 0; e < E; ++e) {
