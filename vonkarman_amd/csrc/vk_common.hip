@@ -191,17 +191,31 @@ int launch_clock_advance(Scalars *sc, hipStream_t st) {
 // The table is indexed by the integer |k|^2 (k2 = kx^2+ky^2+kz^2 is an integer
 // for the 2 pi box), so every mode of a shell shares one entry.
 // ---------------------------------------------------------------------------
-__device__ void phi123(double z, double &p1, double &p2, double &p3) {
+// cs[j] = {cos theta_j, sin theta_j, cos(sin theta_j), sin(sin theta_j)} of
+// the contour points theta_j = 2 pi (j + 1/2) / 32: the parts of each term
+// that do not depend on z (the same device functions as a per-term
+// evaluation, so the terms round identically)
+constexpr int PHI_M = 32;
+__device__ __forceinline__ void phi_points(double (*cs)[4], int tid, int nt) {
     const double PI = 3.14159265358979323846;
-    const int M = 32;
+    for (int j = tid; j < PHI_M; j += nt) {
+        const double theta = 2.0 * PI * ((double)j + 0.5) / (double)PHI_M;
+        const double w_im = 1.0 * sin(theta);
+        cs[j][0] = 1.0 * cos(theta);
+        cs[j][1] = w_im;
+        cs[j][2] = cos(w_im);
+        cs[j][3] = sin(w_im);
+    }
+}
+__device__ void phi123(double z, const double (*cs)[4], double &p1, double &p2, double &p3) {
+    const int M = PHI_M;
     double s1 = 0.0, s2 = 0.0, s3 = 0.0;
     for (int j = 0; j < M; ++j) {
-        const double theta = 2.0 * PI * ((double)j + 0.5) / (double)M;
-        const double w_re = z + 1.0 * cos(theta);
-        const double w_im = 1.0 * sin(theta);
+        const double w_re = z + cs[j][0];
+        const double w_im = cs[j][1];
         const double ex = exp(w_re);
-        const double ew_re = ex * cos(w_im);
-        const double ew_im = ex * sin(w_im);
+        const double ew_re = ex * cs[j][2];
+        const double ew_im = ex * cs[j][3];
         {  // phi1
             const double num_re = ew_re - 1.0, num_im = ew_im;
             const double denom = w_re * w_re + w_im * w_im;
@@ -230,6 +244,9 @@ __device__ void phi123(double z, double &p1, double &p2, double &p3) {
 
 __global__ void k_etd_table(double *tab, int qmax, double nu, double ks, const Scalars *sc, int force) {
     if (!force && !sc->rebuild) return;
+    __shared__ double cs[PHI_M][4];
+    phi_points(cs, threadIdx.x, blockDim.x);
+    __syncthreads();
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q > qmax) return;
     const int S = qmax + 1;
@@ -238,8 +255,8 @@ __global__ void k_etd_table(double *tab, int qmax, double nu, double ks, const S
     const double z = lambda * sc->dt;
     const double zh = z / 2.0;
     double p1h, p2h, p3h, p1, p2, p3;
-    phi123(zh, p1h, p2h, p3h);
-    phi123(z, p1, p2, p3);
+    phi123(zh, cs, p1h, p2h, p3h);
+    phi123(z, cs, p1, p2, p3);
     tab[0 * S + q] = exp(z);
     tab[1 * S + q] = exp(zh);
     tab[2 * S + q] = p1h;

@@ -1331,8 +1331,10 @@ __global__ __launch_bounds__((ZMaxGeom<L, TW>::NT)) void k_z_max(const cpx<TW> *
     constexpr int T = ZMaxGeom<L, TW>::T, NT = ZMaxGeom<L, TW>::NT;
     constexpr int HIN = L / 2 + 1;
     __shared__ cpx<TW> lds[L * T];
+    __shared__ cpx<TW> ltw[L];
     __shared__ double wmax[NT / 64];
     const int tid = threadIdx.x;
+    stage_twiddles<L, NT>(ltw, tw, tid);
     const long long line0 = (long long)blockIdx.x * T;
     const int t = tid % T;
     const bool ok = line0 + t < lines;
@@ -1340,12 +1342,19 @@ __global__ __launch_bounds__((ZMaxGeom<L, TW>::NT)) void k_z_max(const cpx<TW> *
     const long long base = (ok ? line0 + t : lines - 1) * hp;
     cpx<TW> v[E];
     TW r[3][E];
+    // the uz line's loads are in flight during the (ux, uy) pair's transform
+    ZRaw<TW> raw[E];
+    const gptr<cpx<TW>> f2 = opaque(in) + 2 * fs;
+    z_pair_issue<L, HIN, PL::first, T, NT, true>(raw, opaque(in), opaque(in) + fs, base, tid);
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
-        const gptr<cpx<TW>> fa = opaque(in) + (2 * q) * fs;
-        if (q == 0) z_pair_fill<L, HIN, PL::first, T, NT, true>(v, fa, opaque(in) + fs, base, tid);
-        else z_pair_fill<L, HIN, PL::first, T, NT, false>(v, fa, fa, base, tid);
-        fft_reg<L, +1, T, NT, T, 1, PL>(v, lds, tw, tid);
+        if (q == 0) {
+            z_pair_combine<L, HIN, PL::first, T, NT, true>(v, raw, tid);
+            z_pair_issue<L, HIN, PL::first, T, NT, false>(raw, f2, f2, base, tid);
+        } else {
+            z_pair_combine<L, HIN, PL::first, T, NT, false>(v, raw, tid);
+        }
+        fft_reg_lt<L, +1, T, NT, T, 1, PL>(v, lds, ltw, tid);
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             r[2 * q][e] = v[e].re * s_in;
