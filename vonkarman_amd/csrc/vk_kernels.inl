@@ -14,6 +14,8 @@
 #include "vk_fft.hpp"
 #include "vk_launch.h"
 
+#include <type_traits>
+
 namespace vk {
 
 
@@ -85,36 +87,49 @@ constexpr bool pad_stores() {
 // ===========================================================================
 // single-field modes use the low-pass-count plan; modes that combine fields
 // use the smaller-E plan to stay within the register budget
-template <int L, int MODE>
+template <int L, int MODE, typename TW>
 struct InvPlan {
     using type = typename PlanSmall<L>::type;
 };
-template <int L>
-struct InvPlan<L, INV_PLAIN> {
+template <int L, typename TW>
+struct InvPlan<L, INV_PLAIN, TW> {
     using type = PlanS<L>;
 };
-template <int L>
-struct InvPlan<L, INV_U3> {
+template <int L, typename TW>
+struct InvPlan<L, INV_U3, TW> {
     using type = PlanS<L>;
 };
-// The padded x pass runs the 12-element plan in reversed radix order, the
-// order of the fused stage pass's inverse (k_fwd_stage_inv).
+// The stage passes' x forward plan (k_fwd FWD_STAGE and the fused pass).  FP64
+// uses the 12-element plan (2 waves per SIMD at 256 VGPRs); FP32 uses the
+// low-pass-count plan (24 elements, radices 8 8 6 at 384): its 16-column
+// tiles then make 4-wave blocks instead of 8-wave ones, so two blocks share a
+// CU, and one LDS exchange fewer per transform.
+#ifndef VK_F32_STAGE_E24
+#define VK_F32_STAGE_E24 1
+#endif
+template <int L, typename TW>
+struct StagePlan {
+    using type = std::conditional_t<sizeof(TW) == 4 && VK_F32_STAGE_E24, PlanS<L>, PlanE12<L>>;
+};
+// The padded x inverse runs the stage plan in reversed radix order, the
+// order of the fused stage pass's inverse (k_fwd_stage_inv), so the fused and
+// unfused paths round identically.
 template <typename PLAN>
 struct ReversedPlan {
     using radices = typename Reverse<typename PLAN::radices>::type;
     static constexpr int E = PLAN::E;
 };
-template <int L, bool PADDED = (L % 3 == 0)>
+template <int L, typename TW, bool PADDED = (L % 3 == 0)>
 struct XPassPlan {
-    using type = ReversedPlan<PlanE12<L>>;
+    using type = ReversedPlan<typename StagePlan<L, TW>::type>;
 };
-template <int L>
-struct XPassPlan<L, false> {
+template <int L, typename TW>
+struct XPassPlan<L, TW, false> {
     using type = typename PlanSmall<L>::type;
 };
-template <int L>
-struct InvPlan<L, INV_U3DX> {
-    using type = typename XPassPlan<L>::type;
+template <int L, typename TW>
+struct InvPlan<L, INV_U3DX, TW> {
+    using type = typename XPassPlan<L, TW>::type;
 };
 // bytes of each position row of a strided tile (T = row bytes / element bytes columns)
 #ifndef VK_ROW_BYTES_INV
@@ -130,20 +145,21 @@ struct InvPlan<L, INV_U3DX> {
 #define VK_ROW_BYTES_FWD_F32 VK_ROW_BYTES_FWD
 #endif
 template <int L, int MODE, typename TW>
-using InvGeom = Geom<L, InvPlan<L, MODE>::type::E, TW, sizeof(TW) == 8 ? VK_ROW_BYTES_INV : VK_ROW_BYTES_INV_F32>;
+using InvGeom = Geom<L, InvPlan<L, MODE, TW>::type::E, TW, sizeof(TW) == 8 ? VK_ROW_BYTES_INV : VK_ROW_BYTES_INV_F32>;
 
 template <int L, int LIN, int MODE, typename TI, typename TW>
 __global__ __launch_bounds__((InvGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_waves_per_eu(Waves<K_INV, TW>::value))) void k_inv(const cpx<TI> *__restrict__ in,
                                                               cpx<TW> *__restrict__ out,
                                                               const cpx<TW> *__restrict__ tw, Strided g, int n,
                                                               double ks, TW scale) {
-    using P = typename InvPlan<L, MODE>::type;
+    using P = typename InvPlan<L, MODE, TW>::type;
     using PL = typename P::radices;
     constexpr int E = P::E;
     constexpr int T = InvGeom<L, MODE, TW>::T, NT = InvGeom<L, MODE, TW>::NT;
     // prefetch the next input field while transforming (multi-field modes with
-    // the 12-element plan: a 24-element prefetch buffer would cost occupancy)
-    constexpr bool PF = MODE != INV_PLAIN && E <= 12;
+    // the 12-element plan, and the FP32 x pass: a 24-element FP64 prefetch
+    // buffer would cost occupancy)
+    constexpr bool PF = MODE != INV_PLAIN && (E <= 12 || (sizeof(TW) == 4 && MODE == INV_U3DX));
     __shared__ cpx<TW> lds[L * T];
     __shared__ cpx<TW> ltw[PF ? L : 1];
     const int tid = threadIdx.x;
@@ -594,27 +610,27 @@ __device__ __forceinline__ void adj_chunk(const StageArgs &sa, bool st, const lo
     }
 }
 
-template <int L, int MODE>
+template <int L, int MODE, typename TW>
 struct FwdPlan {
     using type = PlanS<L>;
 };
-template <int L>
-struct FwdPlan<L, FWD_STAGE> {
-    using type = PlanE12<L>;
+template <int L, typename TW>
+struct FwdPlan<L, FWD_STAGE, TW> {
+    using type = typename StagePlan<L, TW>::type;
 };
-template <int L>
-struct FwdPlan<L, FWD_VJP> {
+template <int L, typename TW>
+struct FwdPlan<L, FWD_VJP, TW> {
     using type = PlanE12<L>;
 };
 template <int L, int MODE, typename TW>
-using FwdGeom = Geom<L, FwdPlan<L, MODE>::type::E, TW, sizeof(TW) == 8 ? VK_ROW_BYTES_FWD : VK_ROW_BYTES_FWD_F32>;
+using FwdGeom = Geom<L, FwdPlan<L, MODE, TW>::type::E, TW, sizeof(TW) == 8 ? VK_ROW_BYTES_FWD : VK_ROW_BYTES_FWD_F32>;
 
 template <int L, int LOUT, int MODE, int NF, int STG, typename TW>
 __global__ __launch_bounds__((FwdGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_waves_per_eu(Waves<MODE == FWD_VJP ? K_FVJP : K_FWD, TW>::value))) void k_fwd(const cpx<TW> *__restrict__ in,
                                                                     cpx<TW> *__restrict__ out,
                                                                     const cpx<TW> *__restrict__ tw, Strided g,
                                                                     int n, double ks, TW scale, StageArgs sa) {
-    using P = typename FwdPlan<L, MODE>::type;
+    using P = typename FwdPlan<L, MODE, TW>::type;
     using PL = typename P::radices;
     constexpr int E = P::E;
     constexpr int T = FwdGeom<L, MODE, TW>::T, NT = FwdGeom<L, MODE, TW>::NT;
@@ -827,7 +843,7 @@ template <int L, int LOUT, int STG, typename TW>
 __global__ __launch_bounds__((FwdGeom<L, FWD_STAGE, TW>::NT)) __attribute__((amdgpu_waves_per_eu(Waves<K_FWD, TW>::value))) void k_fwd_stage_inv(
     const cpx<TW> *__restrict__ in, cpx<TW> *__restrict__ aout, const cpx<TW> *__restrict__ tw, Strided g, Strided gi,
     int n, double ks, StageArgs sa) {
-    using P = PlanE12<L>;
+    using P = typename StagePlan<L, TW>::type;
     using PL = typename P::radices;         // forward order
     using RL = typename Reverse<PL>::type;  // inverse order, first radix = forward's last
     constexpr int E = P::E;
