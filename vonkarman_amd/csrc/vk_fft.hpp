@@ -319,6 +319,12 @@ template <int L, int R1, int T, int NT>
 constexpr int ci_in_base(int e) {
     return (e / R1) * (NT / T) + (e % R1) * (L / R1);
 }
+// the same for the output positions of a transform whose last radix is RP
+// (ci_pos_out): [ci_out_base(e), ci_out_base(e) + NT/T)
+template <int L, int RP, int T, int NT>
+constexpr int ci_out_base(int e) {
+    return (e / RP) * (NT / T) + (e % RP) * (L / RP);
+}
 
 // Positions of the (3/2-rule) zero gap [lin/2, L - lin/2) of a padded line
 // relative to the range [b, b + w): 0 = entirely outside the gap (every
@@ -525,7 +531,10 @@ __device__ __forceinline__ void fft_reg_lt(cpx<R> (&v)[E], cpx<R> *lds, const cp
 // (the first transform's barrier comes after its first pass, which reads no
 // twiddles, so a __syncthreads() before the first transform is enough)
 // All loads are issued before the first LDS write (a load feeding a write in
-// the same loop trip is waited for before the next trip's load is issued).
+// the same loop trip is waited for before the next trip's load is issued),
+// and no write is predicated (the compiler would sink its load into the
+// branch): threads past the end load and write entry L-1, the value its own
+// thread writes.
 template <int L, int NT, typename R>
 __device__ __forceinline__ void stage_twiddles(cpx<R> *ltw, const cpx<R> *__restrict__ tw, int tid) {
     const gptr<cpx<R>> g = (gptr<cpx<R>>)tw;
@@ -539,7 +548,7 @@ __device__ __forceinline__ void stage_twiddles(cpx<R> *ltw, const cpx<R> *__rest
 #pragma unroll
     for (int c = 0; c < C; ++c) {
         const int i = tid + c * NT;
-        if (L % NT == 0 || i < L) ltw[i] = t[c];
+        ltw[i < L ? i : L - 1] = t[c];
     }
 }
 #endif

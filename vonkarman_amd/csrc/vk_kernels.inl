@@ -99,6 +99,15 @@ template <int L, typename TW>
 struct InvPlan<L, INV_U3, TW> {
     using type = PlanS<L>;
 };
+// the FP32 y inverse runs the 24-element plan too (4-wave blocks of 16
+// columns; 0.362 -> 0.339 ms at 384), and prefetches like the 12-element one
+#ifndef VK_F32_YINV_E24
+#define VK_F32_YINV_E24 1
+#endif
+template <int L, typename TW>
+struct InvPlan<L, INV_UOMEGA, TW> {
+    using type = std::conditional_t<sizeof(TW) == 4 && VK_F32_YINV_E24, PlanS<L>, typename PlanSmall<L>::type>;
+};
 // The stage passes' x forward plan (k_fwd FWD_STAGE and the fused pass).  FP64
 // uses the 12-element plan (2 waves per SIMD at 256 VGPRs); FP32 uses the
 // low-pass-count plan (24 elements, radices 8 8 6 at 384): its 16-column
@@ -159,7 +168,7 @@ __global__ __launch_bounds__((InvGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
     // prefetch the next input field while transforming (multi-field modes with
     // the 12-element plan, and the FP32 x pass: a 24-element FP64 prefetch
     // buffer would cost occupancy)
-    constexpr bool PF = MODE != INV_PLAIN && (E <= 12 || (sizeof(TW) == 4 && MODE == INV_U3DX));
+    constexpr bool PF = MODE != INV_PLAIN && (E <= 12 || (sizeof(TW) == 4 && (MODE == INV_U3DX || MODE == INV_UOMEGA)));
     __shared__ cpx<TW> lds[L * T];
     __shared__ cpx<TW> ltw[PF ? L : 1];
     const int tid = threadIdx.x;
@@ -812,17 +821,30 @@ __global__ __launch_bounds__((FwdGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
             stage_chunk<STG, CH, TW>(sa, colok, idx, q, nn, nullptr, k2d);
         }
     } else {
+        // Every column stores (padding columns into row padding, see
+        // pad_stores), and whether an element's outputs are kept under the
+        // truncation is decided at compile time wherever its whole position
+        // range is kept or dropped: no store sits under a per-lane branch.
+        constexpr bool PADP = pad_stores<TW, T>();
+        const bool stp = PADP || colok;
+        const long long coutp = stp ? col_addr(col, g.out_inner, g.out_os) : 0;
+        auto store_field = [&](int f) {
+            if (stp) {
+                static_for<E>([&](auto ec) {
+                    constexpr int e = decltype(ec)::value;
+                    constexpr int cls = LOUT == L ? 0 : gap_class(ci_out_base<L, RP, T, NT>(e), NT / T, LOUT, L);
+                    if constexpr (cls != 1) {
+                        const int p = ci_pos_out<L, RP, T, NT>(tid, e);
+                        const int j = LOUT == L ? p : unpad_index(p, LOUT, L);
+                        if (cls == 0 || j >= 0) out[f * g.out_fs + coutp + (long long)j * g.out_ls] = cscale(v[e], scale);
+                    }
+                });
+            }
+        };
         for (int f = 0; f < NF; ++f) {
             load_field(f);
             fft_reg<L, -1, T, NT, T, 1, PL>(v, lds, tw, tid);
-            if (colok) {
-#pragma unroll
-                for (int e = 0; e < E; ++e) {
-                    const int p = ci_pos_out<L, RP, T, NT>(tid, e);
-                    const int j = LOUT == L ? p : unpad_index(p, LOUT, L);
-                    if (j >= 0) out[f * g.out_fs + cout + (long long)j * g.out_ls] = cscale(v[e], scale);
-                }
-            }
+            store_field(f);
             __syncthreads();
         }
     }
@@ -1536,6 +1558,7 @@ template <int L, int LOUT, int MODE, int NF, int STG, typename TW>
 int run_fwd(const void *in, void *out, const void *tw, const Strided &g, int n, double ks, double scale,
             const StageArgs *sa, hipStream_t st) {
     using C = FwdGeom<L, MODE, TW>;
+    if (MODE == FWD_PLAIN && pad_stores<TW, C::T>() && g.ncols % C::T != 0) return 2;
     const unsigned nb = (unsigned)((g.ncols + C::T - 1) / C::T);
     StageArgs a{};
     if (sa) a = *sa;
