@@ -511,7 +511,7 @@ int vk_solver_create(int n, double l, double nu, int precision, vk_solver **out)
         (r = dev_alloc(s->n1, state_bytes, s->total)) || (r = dev_alloc(s->sacc, state_bytes, s->total)) ||
         (r = dev_alloc(s->A, a_elems * s->wbytes(), s->total)) ||
         (r = dev_alloc(s->B, b_elems * s->wbytes(), s->total)) ||
-        (r = dev_alloc(s->tab, (size_t)7 * (s->qmax + 1) * sizeof(double), s->total)) ||
+        (r = dev_alloc(s->tab, (size_t)ETD_REC * (s->qmax + 1) * sizeof(double), s->total)) ||
         (r = dev_alloc(s->sc, sizeof(Scalars), s->total)) ||
         (r = upload_twiddles(s->tw_n, n, s->wp, s->total)) ||
         (r = upload_twiddles(s->tw_m, s->m, s->wp, s->total)) ||

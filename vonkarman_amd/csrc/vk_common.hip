@@ -249,7 +249,6 @@ __global__ void k_etd_table(double *tab, int qmax, double nu, double ks, const S
     __syncthreads();
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q > qmax) return;
-    const int S = qmax + 1;
     const double k2 = (double)q * (ks * ks);
     const double lambda = -nu * k2;
     const double z = lambda * sc->dt;
@@ -257,13 +256,14 @@ __global__ void k_etd_table(double *tab, int qmax, double nu, double ks, const S
     double p1h, p2h, p3h, p1, p2, p3;
     phi123(zh, cs, p1h, p2h, p3h);
     phi123(z, cs, p1, p2, p3);
-    tab[0 * S + q] = exp(z);
-    tab[1 * S + q] = exp(zh);
-    tab[2 * S + q] = p1h;
-    tab[3 * S + q] = p1;
-    tab[4 * S + q] = p1 - 3.0 * p2 + 4.0 * p3;
-    tab[5 * S + q] = 2.0 * p2 - 4.0 * p3;
-    tab[6 * S + q] = -p2 + 4.0 * p3;
+    tab[etd_index(ETD_E, q)] = exp(z);
+    tab[etd_index(ETD_EH, q)] = exp(zh);
+    tab[etd_index(ETD_A21, q)] = p1h;
+    tab[etd_index(ETD_A41, q)] = p1;
+    tab[etd_index(ETD_B1, q)] = p1 - 3.0 * p2 + 4.0 * p3;
+    tab[etd_index(ETD_B23, q)] = 2.0 * p2 - 4.0 * p3;
+    tab[etd_index(ETD_B4, q)] = -p2 + 4.0 * p3;
+    tab[etd_index(ETD_B4, q) + 1] = 0.0;
 }
 int launch_etd_table(double *tab, int qmax, double nu, double ks, const Scalars *sc, int force, hipStream_t st) {
     const unsigned nb = (unsigned)((qmax + 1 + 127) / 128);
@@ -402,7 +402,7 @@ __global__ void k_adj_prep(const cpx<double> *__restrict__ lam, cpx<double> *__r
         double coef = 1.0;
         if (use_b4) {
             const int wx = wavenum(ix, n), wy = wavenum(iy, n);
-            coef = sc->dt * tab[6 * ts + wx * wx + wy * wy + iz * iz];
+            coef = sc->dt * tab[etd_index(ETD_B4, wx * wx + wy * wy + iz * iz)];
         }
         cpx<double> x[3];
         for (int f = 0; f < 3; ++f) x[f] = {coef * lam[f * fs + i].re, coef * lam[f * fs + i].im};

@@ -320,8 +320,8 @@ __global__ __launch_bounds__((InvGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
 // read column 0); only the stores are predicated.
 // ===========================================================================
 // ETD-RK4 stage epilogue (vonkarman-periodic/src/solver.rs:264-378) for CH
-// modes; nn = Leray-projected N-hat of the modes.  Table rows: 0 e, 1 e_half,
-// 2 a21 (= a31), 3 a41, 4 b1, 5 b23, 6 b4.
+// modes; nn = Leray-projected N-hat of the modes.  Table rows (etd_index):
+// e, e_half, a21 (= a31), a41, b1, b23, b4.
 // XO: also return the stage output (the next stage input, or the new state)
 // in xo; it is stored to sa.out only when sa.out is set (uniform branch).
 // k2[i]: |k|^2 of mode i in FP64 (kx^2 + ky^2 + kz^2, as SpectralOps::k_mag_sq),
@@ -335,7 +335,6 @@ __device__ __forceinline__ void stage_chunk(const StageArgs &sa, bool st, const 
     cpx<TS> *n1 = reinterpret_cast<cpx<TS> *>(sa.n1);
     cpx<TS> *sacc = reinterpret_cast<cpx<TS> *>(sa.sacc);
     const double *tab = sa.tab;
-    const int ts = sa.tab_stride;
     const long long fs = sa.fs;
     if constexpr (STG == ST_RAW) {
         if (st) {
@@ -393,8 +392,8 @@ __device__ __forceinline__ void stage_chunk(const StageArgs &sa, bool st, const 
         cpx<TS> uu[CH][3];
 #pragma unroll
         for (int i = 0; i < CH; ++i) {
-            eh[i] = tab[1 * ts + q[i]];
-            a[i] = tab[2 * ts + q[i]];
+            eh[i] = tab[etd_index(ETD_EH, q[i])];
+            a[i] = tab[etd_index(ETD_A21, q[i])];
 #pragma unroll
             for (int f = 0; f < 3; ++f) uu[i][f] = u[f * fs + idx[i]];
         }
@@ -424,10 +423,10 @@ __device__ __forceinline__ void stage_chunk(const StageArgs &sa, bool st, const 
         cpx<TS> uu[CH][3], s2[CH][3], m1[CH][3];
 #pragma unroll
         for (int i = 0; i < CH; ++i) {
-            e[i] = tab[0 * ts + q[i]];
-            a[i] = tab[3 * ts + q[i]];
-            b1[i] = tab[4 * ts + q[i]];
-            b23[i] = tab[5 * ts + q[i]];
+            e[i] = tab[etd_index(ETD_E, q[i])];
+            a[i] = tab[etd_index(ETD_A41, q[i])];
+            b1[i] = tab[etd_index(ETD_B1, q[i])];
+            b23[i] = tab[etd_index(ETD_B23, q[i])];
 #pragma unroll
             for (int f = 0; f < 3; ++f) {
                 uu[i][f] = u[f * fs + idx[i]];
@@ -464,7 +463,7 @@ __device__ __forceinline__ void stage_chunk(const StageArgs &sa, bool st, const 
         cpx<TS> p[CH][3];
 #pragma unroll
         for (int i = 0; i < CH; ++i) {
-            b4[i] = tab[6 * ts + q[i]];
+            b4[i] = tab[etd_index(ETD_B4, q[i])];
 #pragma unroll
             for (int f = 0; f < 3; ++f) p[i][f] = sacc[f * fs + idx[i]];
         }
@@ -512,7 +511,6 @@ __device__ __forceinline__ void adj_chunk(const StageArgs &sa, bool st, const lo
     cpx<double> *cot = reinterpret_cast<cpx<double> *>(sa.cot);
     cpx<double> *out = reinterpret_cast<cpx<double> *>(sa.out);
     const double *tab = sa.tab;
-    const int ts = sa.tab_stride;
     if constexpr (STG == AS_RAW) {
         if (st) {
 #pragma unroll
@@ -543,10 +541,10 @@ __device__ __forceinline__ void adj_chunk(const StageArgs &sa, bool st, const lo
             cpx<double> l[CH][3];
 #pragma unroll
             for (int i = 0; i < CH; ++i) {
-                e[i] = tab[0 * ts + q[i]];
-                a41[i] = tab[3 * ts + q[i]];
-                b1[i] = tab[4 * ts + q[i]];
-                b23[i] = tab[5 * ts + q[i]];
+                e[i] = tab[etd_index(ETD_E, q[i])];
+                a41[i] = tab[etd_index(ETD_A41, q[i])];
+                b1[i] = tab[etd_index(ETD_B1, q[i])];
+                b23[i] = tab[etd_index(ETD_B23, q[i])];
 #pragma unroll
                 for (int f = 0; f < 3; ++f) l[i][f] = lam[f * fs + idx[i]];
             }
@@ -567,9 +565,9 @@ __device__ __forceinline__ void adj_chunk(const StageArgs &sa, bool st, const lo
             cpx<double> l[CH][3], g[CH][3];
 #pragma unroll
             for (int i = 0; i < CH; ++i) {
-                eh[i] = tab[1 * ts + q[i]];
-                a31[i] = tab[2 * ts + q[i]];
-                b23[i] = tab[5 * ts + q[i]];
+                eh[i] = tab[etd_index(ETD_EH, q[i])];
+                a31[i] = tab[etd_index(ETD_A21, q[i])];
+                b23[i] = tab[etd_index(ETD_B23, q[i])];
 #pragma unroll
                 for (int f = 0; f < 3; ++f) {
                     l[i][f] = lam[f * fs + idx[i]];
@@ -590,8 +588,8 @@ __device__ __forceinline__ void adj_chunk(const StageArgs &sa, bool st, const lo
             cpx<double> g[CH][3], p[CH][3];
 #pragma unroll
             for (int i = 0; i < CH; ++i) {
-                eh[i] = tab[1 * ts + q[i]];
-                a21[i] = tab[2 * ts + q[i]];
+                eh[i] = tab[etd_index(ETD_EH, q[i])];
+                a21[i] = tab[etd_index(ETD_A21, q[i])];
 #pragma unroll
                 for (int f = 0; f < 3; ++f) {
                     g[i][f] = gu[f * fs + idx[i]];

@@ -28,6 +28,14 @@ struct Strided {
     long long out_os, out_ls, out_fs;
 };
 
+// ETD-RK4 coefficients by integer |k|^2 = q: one record of ETD_REC doubles per
+// q, so the coefficients a mode's stage reads share one 64-B line (a mode
+// reads 1 to 4 of them; rows that are read together are adjacent and
+// 16-B aligned).
+enum EtdRow { ETD_EH = 0, ETD_A21 = 1, ETD_E = 2, ETD_A41 = 3, ETD_B1 = 4, ETD_B23 = 5, ETD_B4 = 6 };
+constexpr int ETD_REC = 8;
+__host__ __device__ inline long long etd_index(int row, int q) { return (long long)q * ETD_REC + row; }
+
 // Device scalars of one solver (one allocation, read by kernels in place so a
 // time step never synchronises with the host).
 struct Scalars {
@@ -76,7 +84,7 @@ struct StageArgs {
     void *sacc;       // N2 (ST_2), then e u + dt (b1 N1 + b23 (N2+N3)) (ST_3), read by ST_4;
                       // RK4: k1 + 2 k2 (+ 2 k3)
     long long fs;     // field stride of the state-layout arrays
-    const double *tab;  // ETD table, 7 arrays of (qmax+1) doubles
+    const double *tab;  // ETD table: (qmax+1) records of ETD_REC doubles (etd_index)
     int tab_stride;     // qmax+1
     const Scalars *sc;  // dt
     // reverse sweep (AS_*, complex double in both precisions): lam = cotangent of the step output, gu = cotangent
