@@ -311,6 +311,18 @@ struct vk_solver {
     DevBuf staging;         // unpadded [3][n][n][h] complex
     size_t total = 0;
     hipEvent_t ev[8] = {};
+    // adaptive steps: the CFL pass of u (x, y inverse, z max, dt, ETD table)
+    // runs on a second stream into its own work arrays, concurrently with
+    // the first stage's x, y, z and y-forward passes, which need no dt; the
+    // first stage epilogue waits for it (ev_cfl).  ev_state marks the state
+    // of the step start on the main stream.
+#ifndef VK_OVERLAP_CFL
+#define VK_OVERLAP_CFL 1
+#endif
+    bool overlap_cfl = VK_OVERLAP_CFL;
+    hipStream_t st2 = nullptr;
+    hipEvent_t ev_state = nullptr, ev_cfl = nullptr;
+    DevBuf C1, C2;  // CFL work arrays, state layout (allocated on first use)
     // discrete adjoint (allocated on first use)
     bool vjp_ready = false;
     DevBuf sa_, sb_, sc_, lam, gu, g1p, cot;  // stage states a, b, c and cotangents
@@ -342,7 +354,7 @@ namespace {
 //   bbuf:    the y-pass output (u, w in y-physical space, 6 fields) goes to
 //            bbuf (field stride M M Hp) instead of the work array B.
 int nl_eval(vk_solver *s, const void *src, StageArgs &sa, vk_timing *t, hipEvent_t *ev, bool skip_p1 = false,
-            bool fuse = false, void *bbuf = nullptr) {
+            bool fuse = false, void *bbuf = nullptr, hipEvent_t wait_stage = nullptr) {
     const int N = s->n, M = s->m, Hp = s->hp, H = s->h;
     const long long NMH = (long long)N * M * Hp, MMH = (long long)M * M * Hp, NH = (long long)N * Hp;
     const long long MH = (long long)M * Hp;
@@ -373,6 +385,8 @@ int nl_eval(vk_solver *s, const void *src, StageArgs &sa, vk_timing *t, hipEvent
     // P1': x forward, truncate, Leray, stage epilogue: D (B) -> state arrays
     // (fused: and the next evaluation's P1 of the stage output -> A)
     Strided g4 = make_strided((int)NH, Hp, H, Hp, MH, Hp, NMH, (int)NH, 0, NH, s->S);
+    // the stage epilogue reads dt and the ETD table
+    if (wait_stage) VK_HIP(hipStreamWaitEvent(s->st, wait_stage, 0));
     if (fuse)
         VK_TRY(launch_fwd_inv(M, N, s->wp, s->B.p, s->A.p, s->tw_m.p, g4, g1, N, s->ks, &sa, s->st));
     else
@@ -382,18 +396,21 @@ int nl_eval(vk_solver *s, const void *src, StageArgs &sa, vk_timing *t, hipEvent
 }
 
 // max |v|^2 over the unpadded physical grid of u (omega=false) or curl u.
-int max_pass(vk_solver *s, bool omega) {
+// Work arrays t1, t2 (state layout) and the stream default to A, B and the
+// solver's stream.
+int max_pass(vk_solver *s, bool omega, hipStream_t st = nullptr, void *t1 = nullptr, void *t2 = nullptr) {
     const int N = s->n, Hp = s->hp, H = s->h;
     const long long NH = (long long)N * Hp, NNH = (long long)N * N * Hp;
-    void *t1 = s->A.p, *t2 = s->B.p;
-    VK_TRY(launch_fill_u64(&s->dsc()->umax2_bits, 0ull, 1, s->st));
+    if (!st) st = s->st;
+    if (!t1) t1 = s->A.p;
+    if (!t2) t2 = s->B.p;
+    VK_TRY(launch_fill_u64(&s->dsc()->umax2_bits, 0ull, 1, st));
     Strided g1 = make_strided((int)NH, Hp, H, (int)NH, 0, NH, s->S, Hp, NH, Hp, NNH);
-    VK_TRY(launch_inv(omega ? INV_U3DX : INV_U3, N, N, s->wp, s->wp, s->u.p, t1, s->tw_n.p, g1, N, s->ks, 1.0,
-                      s->st));
+    VK_TRY(launch_inv(omega ? INV_U3DX : INV_U3, N, N, s->wp, s->wp, s->u.p, t1, s->tw_n.p, g1, N, s->ks, 1.0, st));
     Strided g2 = make_strided((int)NH, Hp, H, (int)NH, 0, NH, NNH, Hp, NH, Hp, NNH);
-    VK_TRY(launch_inv(omega ? INV_OMEGA : INV_U3, N, N, s->wp, s->wp, t1, t2, s->tw_n.p, g2, N, s->ks, 1.0, s->st));
+    VK_TRY(launch_inv(omega ? INV_OMEGA : INV_U3, N, N, s->wp, s->wp, t1, t2, s->tw_n.p, g2, N, s->ks, 1.0, st));
     const double nt = (double)N * N * N;
-    VK_TRY(launch_z_max(N, s->wp, t2, (long long)N * N, Hp, NNH, 1.0 / nt, s->tw_n.p, s->dsc(), s->st));
+    VK_TRY(launch_z_max(N, s->wp, t2, (long long)N * N, Hp, NNH, 1.0 / nt, s->tw_n.p, s->dsc(), st));
     return 0;
 }
 
@@ -428,9 +445,34 @@ int check(vk_solver *s) {
     return 0;
 }
 
+// the CFL pass of the step start on the second stream (see vk_solver::overlap_cfl)
+int cfl_async(vk_solver *s) {
+    if (!s->st2) {
+        VK_HIP(hipStreamCreateWithFlags(&s->st2, hipStreamNonBlocking));
+        VK_HIP(hipEventCreateWithFlags(&s->ev_state, hipEventDisableTiming));
+        VK_HIP(hipEventCreateWithFlags(&s->ev_cfl, hipEventDisableTiming));
+    }
+    if (!s->C1.p) {
+        if (int r = dev_alloc(s->C1, s->state_bytes(), s->total)) return r;
+        if (int r = dev_alloc(s->C2, s->state_bytes(), s->total)) return r;
+    }
+    VK_HIP(hipEventRecord(s->ev_state, s->st));
+    VK_HIP(hipStreamWaitEvent(s->st2, s->ev_state, 0));
+    if (int r = max_pass(s, false, s->st2, s->C1.p, s->C2.p)) return r;
+    VK_TRY(launch_cfl_update(s->dsc(), s->dx, s->nu, 0.5, 0, s->st2));
+    VK_TRY(launch_etd_table(static_cast<double *>(s->tab.p), s->qmax, s->nu, s->ks, s->dsc(), 0, s->st2));
+    VK_HIP(hipEventRecord(s->ev_cfl, s->st2));
+    return 0;
+}
+
 int one_step(vk_solver *s, vk_timing *t) {
     float ms = 0;
-    if (!s->fixed) {
+    // untimed adaptive steps overlap the CFL pass with the first stage's
+    // passes before its epilogue; timed steps run it alone (its own time)
+    const bool overlap = !s->fixed && !t && s->overlap_cfl;
+    if (overlap) {
+        if (int r = cfl_async(s)) return r;
+    } else if (!s->fixed) {
         if (t) VK_HIP(hipEventRecord(s->ev[6], s->st));
         if (int r = max_pass(s, false)) return r;
         VK_TRY(launch_cfl_update(s->dsc(), s->dx, s->nu, 0.5, 0, s->st));
@@ -454,7 +496,8 @@ int one_step(vk_solver *s, vk_timing *t) {
         const void *src = k == 0 ? s->u.p : s->x.p;
         void *dst = k == 3 ? s->u.p : (fuse ? nullptr : s->x.p);
         StageArgs sa = stage_args(s, stages[k], dst);
-        if (int r = nl_eval(s, src, sa, t, s->ev, skip, fuse)) return r;
+        if (int r = nl_eval(s, src, sa, t, s->ev, skip, fuse, nullptr, overlap && k == 0 ? s->ev_cfl : nullptr))
+            return r;
         if (t) {
             VK_HIP(hipEventSynchronize(s->ev[5]));
             double *acc[5] = {&t->x_inverse, &t->y_inverse, &t->z_cross, &t->y_forward,
@@ -545,14 +588,18 @@ int vk_solver_create(int n, double l, double nu, int precision, vk_solver **out)
 int vk_solver_destroy(vk_solver *s) {
     if (!s) return VK_OK;
     if (s->st) (void)hipStreamSynchronize(s->st);
+    if (s->st2) (void)hipStreamSynchronize(s->st2);
     for (DevBuf *b : {&s->u, &s->x, &s->n1, &s->sacc, &s->A, &s->B, &s->tab, &s->sc, &s->tw_n, &s->tw_m,
                       &s->partials, &s->spec, &s->staging, &s->sa_, &s->sb_, &s->sc_, &s->lam, &s->gu, &s->g1p,
-                      &s->cot, &s->WA, &s->WB})
+                      &s->cot, &s->WA, &s->WB, &s->C1, &s->C2})
         dev_free(*b);
     for (DevBuf &b : s->snaps) dev_free(b);
     for (DevBuf &b : s->KB) dev_free(b);
     for (auto &e : s->ev)
         if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : {s->ev_state, s->ev_cfl})
+        if (e) (void)hipEventDestroy(e);
+    if (s->st2) (void)hipStreamDestroy(s->st2);
     if (s->st) (void)hipStreamDestroy(s->st);
     delete s;
     return VK_OK;
