@@ -351,6 +351,8 @@ __global__ void k_diag_final(const double *__restrict__ partials, int nblocks, i
     const int ncol = 4 + nshells;
     for (int c = threadIdx.x; c < ncol; c += blockDim.x) {
         Neu a;
+        // block order is kept; the unrolled loads are in flight together
+#pragma unroll 16
         for (int b = 0; b < nblocks; ++b) a.add(partials[(long long)b * ncol + c]);
         const double v = a.val() * inv_nt2;
         if (c < 4) sc->diag[c] = c == 0 ? 0.5 * v : v;
