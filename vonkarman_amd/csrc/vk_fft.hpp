@@ -459,6 +459,21 @@ template <typename R>
 __device__ __forceinline__ cpx<R> ldg(gptr<cpx<R>> p, long long i) {
     return {p[i].re, p[i].im};
 }
+// Streaming (non-temporal) load and store of one complex element, for data a
+// pass reads or writes once: they keep reused lines (the ETD table) from
+// being displaced in the caches.
+template <typename R>
+__device__ __forceinline__ cpx<R> ldg_nt(gptr<cpx<R>> p, long long i) {
+    using V = R __attribute__((ext_vector_type(2)));
+    const V v = __builtin_nontemporal_load(reinterpret_cast<const __attribute__((address_space(1))) V *>(p + i));
+    return {v.x, v.y};
+}
+template <typename R>
+__device__ __forceinline__ void stg_nt(cpx<R> *p, cpx<R> v) {
+    using V = R __attribute__((ext_vector_type(2)));
+    const V x = {v.re, v.im};
+    __builtin_nontemporal_store(x, reinterpret_cast<V *>(p));
+}
 
 // one pass of the register-staged transform
 // (TP: the twiddle table pointer, global or LDS)

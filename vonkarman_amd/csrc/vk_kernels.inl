@@ -859,6 +859,11 @@ __global__ __launch_bounds__((FwdGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
 // Outputs (5 fields, the x pass layout of the next nonlinear evaluation):
 // 0 ux, 1 uy, 2 uz, 3 i kx uz, 4 i kx uy.
 // ===========================================================================
+// the fused pass streams its input fields and its x-inverse outputs
+// (non-temporal), so the ETD table stays cached
+#ifndef VK_NT_FUSED
+#define VK_NT_FUSED 1
+#endif
 template <int L, int LOUT, int STG, typename TW>
 __global__ __launch_bounds__((FwdGeom<L, FWD_STAGE, TW>::NT)) __attribute__((amdgpu_waves_per_eu(Waves<K_FWD, TW>::value))) void k_fwd_stage_inv(
     const cpx<TW> *__restrict__ in, cpx<TW> *__restrict__ aout, const cpx<TW> *__restrict__ tw, Strided g, Strided gi,
@@ -892,7 +897,10 @@ __global__ __launch_bounds__((FwdGeom<L, FWD_STAGE, TW>::NT)) __attribute__((amd
     auto load_into = [&](cpx<TW>(&S)[E], int f) {
         const gptr<cpx<TW>> src = opaque(in) + f * g.in_fs + cin;
 #pragma unroll
-        for (int e = 0; e < E; ++e) S[e] = ldg(src, (long long)ci_pos_in<L, PL::first, T, NT>(tid, e) * g.in_ls);
+        for (int e = 0; e < E; ++e) {
+            const long long i = (long long)ci_pos_in<L, PL::first, T, NT>(tid, e) * g.in_ls;
+            S[e] = VK_NT_FUSED ? ldg_nt(src, i) : ldg(src, i);
+        }
     };
     // forward transforms of the three fields (each next field loads meanwhile)
     load_into(v, 0);
@@ -966,7 +974,11 @@ __global__ __launch_bounds__((FwdGeom<L, FWD_STAGE, TW>::NT)) __attribute__((amd
         if (PADST || colok) {
 #pragma unroll
             for (int e = 0; e < E; ++e)
-                aout[fo * gi.out_fs + cinv + (long long)ci_pos_out<L, RL::last, T, NT>(tid, e) * gi.out_ls] = v[e];
+            {
+                cpx<TW> *dst = aout + fo * gi.out_fs + cinv + (long long)ci_pos_out<L, RL::last, T, NT>(tid, e) * gi.out_ls;
+                if (VK_NT_FUSED) stg_nt(dst, v[e]);
+                else *dst = v[e];
+            }
         }
         __syncthreads();
     };
