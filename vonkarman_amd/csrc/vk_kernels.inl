@@ -326,6 +326,37 @@ __global__ __launch_bounds__((InvGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
 // in xo; it is stored to sa.out only when sa.out is set (uniform branch).
 // k2[i]: |k|^2 of mode i in FP64 (kx^2 + ky^2 + kz^2, as SpectralOps::k_mag_sq),
 // used by the RK4 stages' viscous term.
+// the stage passes stream their input fields and the fused pass its
+// x-inverse outputs (non-temporal), so the ETD table stays cached
+#ifndef VK_NT_FUSED
+#define VK_NT_FUSED 1
+#endif
+// Streaming (non-temporal) accesses of the state arrays in the stage
+// epilogues: each element is read or written once per pass, and the ETD
+// table read beside them stays cached.
+#ifndef VK_NT_STATE
+#define VK_NT_STATE 1
+#endif
+template <typename R>
+__device__ __forceinline__ cpx<R> ldnt(const cpx<R> *p) {
+    using V = R __attribute__((ext_vector_type(2)));
+    if constexpr (VK_NT_STATE) {
+        const V v = __builtin_nontemporal_load(reinterpret_cast<const V *>(p));
+        return {v.x, v.y};
+    } else {
+        return *p;
+    }
+}
+template <typename R>
+__device__ __forceinline__ void stnt(cpx<R> *p, cpx<R> v) {
+    using V = R __attribute__((ext_vector_type(2)));
+    if constexpr (VK_NT_STATE) {
+        const V x = {v.re, v.im};
+        __builtin_nontemporal_store(x, reinterpret_cast<V *>(p));
+    } else {
+        *p = v;
+    }
+}
 template <int STG, int CH, typename TS, bool XO = false>
 __device__ __forceinline__ void stage_chunk(const StageArgs &sa, bool st, const long long (&idx)[CH],
                                             const int (&q)[CH], const cpx<TS> (&nn)[CH][3],
@@ -341,7 +372,7 @@ __device__ __forceinline__ void stage_chunk(const StageArgs &sa, bool st, const 
 #pragma unroll
             for (int i = 0; i < CH; ++i)
 #pragma unroll
-                for (int f = 0; f < 3; ++f) out[f * fs + idx[i]] = nn[i][f];
+                for (int f = 0; f < 3; ++f) stnt(out + f * fs + idx[i], nn[i][f]);
         }
     } else if constexpr (STG >= RK_1 && STG <= RK_4) {
         // classical RK4 (vonkarman-periodic/src/rk4.rs:30-143): r = N(state) -
@@ -355,10 +386,10 @@ __device__ __forceinline__ void stage_chunk(const StageArgs &sa, bool st, const 
         for (int i = 0; i < CH; ++i)
 #pragma unroll
             for (int f = 0; f < 3; ++f) {
-                stv[i][f] = xin[f * fs + idx[i]];
-                if constexpr (STG != RK_1) uu[i][f] = u[f * fs + idx[i]];
-                if constexpr (STG == RK_2) ac[i][f] = n1[f * fs + idx[i]];
-                if constexpr (STG == RK_3 || STG == RK_4) ac[i][f] = sacc[f * fs + idx[i]];
+                stv[i][f] = ldnt(xin + f * fs + idx[i]);
+                if constexpr (STG != RK_1) uu[i][f] = ldnt(u + f * fs + idx[i]);
+                if constexpr (STG == RK_2) ac[i][f] = ldnt(n1 + f * fs + idx[i]);
+                if constexpr (STG == RK_3 || STG == RK_4) ac[i][f] = ldnt(sacc + f * fs + idx[i]);
             }
         if (st) {
 #pragma unroll
@@ -370,18 +401,18 @@ __device__ __forceinline__ void stage_chunk(const StageArgs &sa, bool st, const 
                     const double ri = double(nn[i][f].im) + visc * double(stv[i][f].im);
                     const long long j = f * fs + idx[i];
                     if constexpr (STG == RK_1) {
-                        n1[j] = {TS(rr), TS(ri)};
-                        out[j] = {TS(stv[i][f].re + 0.5 * dt * rr), TS(stv[i][f].im + 0.5 * dt * ri)};
+                        stnt(n1 + j, cpx<TS>{TS(rr), TS(ri)});
+                        stnt(out + j, cpx<TS>{TS(stv[i][f].re + 0.5 * dt * rr), TS(stv[i][f].im + 0.5 * dt * ri)});
                     } else if constexpr (STG == RK_2) {
-                        sacc[j] = {TS(ac[i][f].re + 2.0 * rr), TS(ac[i][f].im + 2.0 * ri)};
-                        out[j] = {TS(uu[i][f].re + 0.5 * dt * rr), TS(uu[i][f].im + 0.5 * dt * ri)};
+                        stnt(sacc + j, cpx<TS>{TS(ac[i][f].re + 2.0 * rr), TS(ac[i][f].im + 2.0 * ri)});
+                        stnt(out + j, cpx<TS>{TS(uu[i][f].re + 0.5 * dt * rr), TS(uu[i][f].im + 0.5 * dt * ri)});
                     } else if constexpr (STG == RK_3) {
-                        sacc[j] = {TS(ac[i][f].re + 2.0 * rr), TS(ac[i][f].im + 2.0 * ri)};
-                        out[j] = {TS(uu[i][f].re + dt * rr), TS(uu[i][f].im + dt * ri)};
+                        stnt(sacc + j, cpx<TS>{TS(ac[i][f].re + 2.0 * rr), TS(ac[i][f].im + 2.0 * ri)});
+                        stnt(out + j, cpx<TS>{TS(uu[i][f].re + dt * rr), TS(uu[i][f].im + dt * ri)});
                     } else {
                         const double sixth_dt = dt / 6.0;
-                        out[j] = {TS(uu[i][f].re + sixth_dt * (ac[i][f].re + rr)),
-                                  TS(uu[i][f].im + sixth_dt * (ac[i][f].im + ri))};
+                        stnt(out + j, cpx<TS>{TS(uu[i][f].re + sixth_dt * (ac[i][f].re + rr)),
+                                  TS(uu[i][f].im + sixth_dt * (ac[i][f].im + ri))});
                     }
                 }
             }
@@ -395,7 +426,7 @@ __device__ __forceinline__ void stage_chunk(const StageArgs &sa, bool st, const 
             eh[i] = tab[etd_index(ETD_EH, q[i])];
             a[i] = tab[etd_index(ETD_A21, q[i])];
 #pragma unroll
-            for (int f = 0; f < 3; ++f) uu[i][f] = u[f * fs + idx[i]];
+            for (int f = 0; f < 3; ++f) uu[i][f] = ldnt(u + f * fs + idx[i]);
         }
         if (st) {
 #pragma unroll
@@ -403,7 +434,7 @@ __device__ __forceinline__ void stage_chunk(const StageArgs &sa, bool st, const 
 #pragma unroll
                 for (int f = 0; f < 3; ++f) {
                     const cpx<TS> x = nn[i][f];
-                    (STG == ST_1 ? n1 : sacc)[f * fs + idx[i]] = x;
+                    stnt((STG == ST_1 ? n1 : sacc) + f * fs + idx[i], x);
                 }
         }
 #pragma unroll
@@ -413,7 +444,7 @@ __device__ __forceinline__ void stage_chunk(const StageArgs &sa, bool st, const 
                 const cpx<TS> x = nn[i][f];
                 const cpx<TS> y = {TS(eh[i] * uu[i][f].re + dt * a[i] * x.re), TS(eh[i] * uu[i][f].im + dt * a[i] * x.im)};
                 if constexpr (XO) xo[i][f] = y;
-                if (st && (!XO || out)) out[f * fs + idx[i]] = y;
+                if (st && (!XO || out)) stnt(out + f * fs + idx[i], y);
             }
     } else if constexpr (STG == ST_3) {
         // stage c, and the part of the final combine that does not involve N4:
@@ -429,9 +460,9 @@ __device__ __forceinline__ void stage_chunk(const StageArgs &sa, bool st, const 
             b23[i] = tab[etd_index(ETD_B23, q[i])];
 #pragma unroll
             for (int f = 0; f < 3; ++f) {
-                uu[i][f] = u[f * fs + idx[i]];
-                s2[i][f] = sacc[f * fs + idx[i]];
-                m1[i][f] = n1[f * fs + idx[i]];
+                uu[i][f] = ldnt(u + f * fs + idx[i]);
+                s2[i][f] = ldnt(sacc + f * fs + idx[i]);
+                m1[i][f] = ldnt(n1 + f * fs + idx[i]);
             }
         }
         if (st) {
@@ -443,7 +474,7 @@ __device__ __forceinline__ void stage_chunk(const StageArgs &sa, bool st, const 
                     const cpx<TS> s3 = {s2[i][f].re + x.re, s2[i][f].im + x.im};  // N2 + N3
                     const double rr = b1[i] * m1[i][f].re + b23[i] * s3.re;
                     const double ri = b1[i] * m1[i][f].im + b23[i] * s3.im;
-                    sacc[f * fs + idx[i]] = {TS(e[i] * uu[i][f].re + dt * rr), TS(e[i] * uu[i][f].im + dt * ri)};
+                    stnt(sacc + f * fs + idx[i], cpx<TS>{TS(e[i] * uu[i][f].re + dt * rr), TS(e[i] * uu[i][f].im + dt * ri)});
                 }
         }
 #pragma unroll
@@ -454,7 +485,7 @@ __device__ __forceinline__ void stage_chunk(const StageArgs &sa, bool st, const 
                 const double dr = 2.0 * x.re - m1[i][f].re, di = 2.0 * x.im - m1[i][f].im;
                 const cpx<TS> y = {TS(e[i] * uu[i][f].re + dt * a[i] * dr), TS(e[i] * uu[i][f].im + dt * a[i] * di)};
                 if constexpr (XO) xo[i][f] = y;
-                if (st && (!XO || out)) out[f * fs + idx[i]] = y;
+                if (st && (!XO || out)) stnt(out + f * fs + idx[i], y);
             }
     } else {  // ST_4: final combine into u, from the stage-3 partial in sacc
         static_assert(STG == ST_4, "unknown stage");
@@ -465,7 +496,7 @@ __device__ __forceinline__ void stage_chunk(const StageArgs &sa, bool st, const 
         for (int i = 0; i < CH; ++i) {
             b4[i] = tab[etd_index(ETD_B4, q[i])];
 #pragma unroll
-            for (int f = 0; f < 3; ++f) p[i][f] = sacc[f * fs + idx[i]];
+            for (int f = 0; f < 3; ++f) p[i][f] = ldnt(sacc + f * fs + idx[i]);
         }
 #pragma unroll
         for (int i = 0; i < CH; ++i)
@@ -475,7 +506,7 @@ __device__ __forceinline__ void stage_chunk(const StageArgs &sa, bool st, const 
                 const cpx<TS> y = {TS(double(p[i][f].re) + dt * (b4[i] * x.re)),
                                    TS(double(p[i][f].im) + dt * (b4[i] * x.im))};
                 if constexpr (XO) xo[i][f] = y;
-                if (st && (!XO || out)) out[f * fs + idx[i]] = y;
+                if (st && (!XO || out)) stnt(out + f * fs + idx[i], y);
             }
     }
 }
@@ -675,7 +706,7 @@ __global__ __launch_bounds__((FwdGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const int p = ci_pos_in<L, PL::first, T, NT>(tid, e);
-            S[e] = ldg(src, (long long)p * g.in_ls);
+            S[e] = (MODE == FWD_STAGE && VK_NT_FUSED) ? ldg_nt(src, (long long)p * g.in_ls) : ldg(src, (long long)p * g.in_ls);
         }
     };
     auto load_field = [&](int f) { load_into(v, f); };
@@ -859,11 +890,6 @@ __global__ __launch_bounds__((FwdGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
 // Outputs (5 fields, the x pass layout of the next nonlinear evaluation):
 // 0 ux, 1 uy, 2 uz, 3 i kx uz, 4 i kx uy.
 // ===========================================================================
-// the fused pass streams its input fields and its x-inverse outputs
-// (non-temporal), so the ETD table stays cached
-#ifndef VK_NT_FUSED
-#define VK_NT_FUSED 1
-#endif
 template <int L, int LOUT, int STG, typename TW>
 __global__ __launch_bounds__((FwdGeom<L, FWD_STAGE, TW>::NT)) __attribute__((amdgpu_waves_per_eu(Waves<K_FWD, TW>::value))) void k_fwd_stage_inv(
     const cpx<TW> *__restrict__ in, cpx<TW> *__restrict__ aout, const cpx<TW> *__restrict__ tw, Strided g, Strided gi,
