@@ -340,7 +340,7 @@ def workload_adjoint_tubes(n: int = 128, nsteps: int = 200, ndir: int = 8) -> di
     from vonkarman_amd import GridSpec, IcType, Periodic3D
 
     nu, dt = 1e-3, 2e-3
-    s = Periodic3D(GridSpec.cubic(n), nu, IcType.AntiParallelTubes(), init_dt=False)
+    s = Periodic3D(GridSpec.cubic(n), nu, IcType.LambOseenTubes(), init_dt=False)
     s.set_fixed_dt(dt)
     u0 = s.u_hat()
     s.steps(1)

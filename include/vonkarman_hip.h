@@ -199,9 +199,25 @@ int vk_solver_adjoint(vk_solver *s, int nsteps, int objective, const double *lam
 int vk_solver_adjoint_device(vk_solver *s, int nsteps, int objective, const double *d_lam_T,
                              int max_snapshots, double *J, double *d_grad, int64_t *forward_steps);
 
-/* Host-side Taylor-Green initial velocity (vonkarman-periodic/src/ic/
- * taylor_green.rs:10-38): real[3][n][n][n]. */
+/* Host-side initial velocities on the cubic grid x_i = i*l/n, as
+ * real[3][n][n][n] (the VectorField a reference IC returns; Periodic3D::new,
+ * solver.rs:46-70, then transforms it with r2c_3d).  Each evaluates the
+ * reference's expressions in the same order with glibc's libm. */
+/* taylor_green.rs:10-38 */
 int vk_ic_taylor_green(int n, double l, double *u_phys);
+/* abc.rs:14-42 (IcType::Abc { a, b, c }) */
+int vk_ic_abc(int n, double l, double a, double b, double c, double *u_phys);
+/* kida_pelz.rs:15-48 (IcType::KidaPelz) */
+int vk_ic_kida_pelz(int n, double l, double *u_phys);
+/* anti_parallel.rs:18-104 (IcType::AntiParallelTubes) */
+int vk_ic_anti_parallel_tubes(int n, double l, double circulation, double core_radius, double separation,
+                              double perturbation, double *u_phys);
+/* random_isotropic.rs:18-104 (IcType::RandomIsotropic) up to its final
+ * c2r_3d: the seeded, Leray-projected, energy-scaled spectrum as
+ * complex[3][n][n][n/2+1] (interleaved re, im).  The reference's physical field
+ * is vk_fft_c2r_3d of each component (random_isotropic.rs:105-110).  n even. */
+int vk_ic_random_isotropic_spectrum(int n, double l, double k_peak, double total_energy, uint64_t seed,
+                                    double *u_hat);
 
 #ifdef __cplusplus
 }

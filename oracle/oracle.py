@@ -210,6 +210,144 @@ def taylor_green(n: int, l: float = TWO_PI) -> np.ndarray:
     return out
 
 
+# ---------------------------------------------------------------------------
+# The reference's other initial conditions, restated in Python/numpy for small
+# grids.  sin/cos/exp/sqrt come from Python's math module (the platform libm,
+# as the reference's f64 methods call it) and are evaluated per axis or per
+# point; everything else is elementwise numpy arithmetic (one rounding per
+# operation, in the reference's order).
+# ---------------------------------------------------------------------------
+def _axis(n: int, l: float) -> np.ndarray:
+    return np.array([float(i) * l / float(n) for i in range(n)])  # field.rs: i * l / n
+
+
+def _fn(f, a: np.ndarray) -> np.ndarray:
+    return np.array([f(float(v)) for v in a.ravel()]).reshape(a.shape)
+
+
+def abc_flow(n: int, a: float = 1.0, b: float = 1.0, c: float = 1.0, l: float = TWO_PI) -> np.ndarray:
+    """ic/abc.rs:14-42."""
+    import math
+
+    x = _axis(n, l)
+    s, co = _fn(math.sin, x), _fn(math.cos, x)
+    X, Y, Z = (slice(None), None, None), (None, slice(None), None), (None, None, slice(None))
+    u = np.empty((3, n, n, n))
+    u[0] = a * s[Z] + c * co[Y]
+    u[1] = b * s[X] + a * co[Z]
+    u[2] = c * s[Y] + b * co[X]
+    return u
+
+
+def kida_pelz(n: int, l: float = TWO_PI) -> np.ndarray:
+    """ic/kida_pelz.rs:15-48."""
+    import math
+
+    x = _axis(n, l)
+    s, co, c3 = _fn(math.sin, x), _fn(math.cos, x), _fn(math.cos, 3.0 * x)
+    X, Y, Z = (slice(None), None, None), (None, slice(None), None), (None, None, slice(None))
+    u = np.empty((3, n, n, n))
+    u[0] = s[X] * (c3[Y] * co[Z] - co[Y] * c3[Z])
+    u[1] = s[Y] * (c3[Z] * co[X] - co[Z] * c3[X])
+    u[2] = s[Z] * (c3[X] * co[Y] - co[X] * c3[Y])
+    return u
+
+
+def anti_parallel_tubes(n: int, circulation: float = 1.0, core_radius: float = 0.3, separation: float = 1.0,
+                        perturbation: float = 0.1, l: float = TWO_PI) -> np.ndarray:
+    """ic/anti_parallel.rs:18-104 (per point; small n only)."""
+    import math
+
+    two_pi = 2.0 * math.pi
+    sigma2 = core_radius * core_radius
+    y_c = l / 2.0
+    z_c1, z_c2 = l / 2.0 - separation / 2.0, l / 2.0 + separation / 2.0
+    x = _axis(n, l)
+
+    def f(r2):
+        return circulation / (two_pi * (r2 + 1e-30)) * (1.0 - math.exp(-r2 / sigma2)) if r2 > 1e-20 else 0.0
+
+    u = np.zeros((3, n, n, n))
+    for i in range(n):
+        d = perturbation * math.sin(two_pi * x[i] / l)
+        for j in range(n):
+            dy1, dy2 = x[j] - (y_c + d), x[j] - (y_c - d)
+            for k in range(n):
+                dz1, dz2 = x[k] - z_c1, x[k] - z_c2
+                f1, f2 = f(dy1 * dy1 + dz1 * dz1), f(dy2 * dy2 + dz2 * dz2)
+                u[1, i, j, k] = -(dz1 * f1) + dz2 * f2
+                u[2, i, j, k] = dy1 * f1 - dy2 * f2
+    return u
+
+
+def xorshift64(state: int):
+    """The generator of ic/random_isotropic.rs:30-37: Marsaglia's xorshift64
+    with shifts (13, 7, 17) on a u64 state; yields the new state."""
+    m = (1 << 64) - 1
+    while True:
+        state ^= (state << 13) & m
+        state ^= state >> 7
+        state ^= (state << 17) & m
+        yield state
+
+
+def random_isotropic_spectrum(n: int, k_peak: float = 4.0, energy: float = 0.5, seed: int = 42,
+                              l: float = TWO_PI) -> np.ndarray:
+    """ic/random_isotropic.rs:18-104, before its C2R (per mode; small n only)."""
+    import math
+
+    h = n // 2 + 1
+    sc = 2.0 * math.pi / l
+    kf = [float(i if i < n // 2 else i - n) * sc for i in range(n)]
+    kh = [float(i) * sc for i in range(h)]
+    rng = xorshift64(seed)
+    u64max = float(2 ** 64)  # u64::MAX as f64
+    uh = np.zeros((3, n, n, h), np.complex128)
+    for ix in range(n):
+        for iy in range(n):
+            for iz in range(h):
+                km = math.sqrt(kf[ix] * kf[ix] + kf[iy] * kf[iy] + kh[iz] * kh[iz])
+                if km < 1e-10:
+                    continue
+                k2 = km * km
+                amp = math.sqrt(k2 * k2 * math.exp(-km * km / (2.0 * k_peak * k_peak)))
+                for c in range(3):
+                    ph = 2.0 * math.pi * (float(next(rng)) / u64max)
+                    uh[c, ix, iy, iz] = complex(amp * math.cos(ph), amp * math.sin(ph))
+    # Leray projection (vonkarman-core/src/spectral_ops.rs:114-151), same order
+    for ix in range(n):
+        for iy in range(n):
+            for iz in range(h):
+                kx, ky, kz = kf[ix], kf[iy], kh[iz]
+                k2 = kx * kx + ky * ky + kz * kz
+                if k2 < 1e-30:
+                    continue
+                inv = 1.0 / k2
+                ux, uy, uz = uh[:, ix, iy, iz]
+                dr = kx * ux.real + ky * uy.real + kz * uz.real
+                di = kx * ux.imag + ky * uy.imag + kz * uz.imag
+                uh[0, ix, iy, iz] = complex(ux.real - kx * dr * inv, ux.imag - kx * di * inv)
+                uh[1, ix, iy, iz] = complex(uy.real - ky * dr * inv, uy.imag - ky * di * inv)
+                uh[2, ix, iy, iz] = complex(uz.real - kz * dr * inv, uz.imag - kz * di * inv)
+    e = 0.0
+    for c in range(3):
+        for ix in range(n):
+            for iy in range(n):
+                for iz in range(h):
+                    v = uh[c, ix, iy, iz]
+                    w = 1.0 if (iz == 0 or iz == n // 2) else 2.0
+                    e += w * (v.real * v.real + v.imag * v.imag)
+    nt = float(n * n * n)
+    e *= 0.5 / (nt * nt)
+    if e > 1e-30:
+        s = math.sqrt(energy / e)
+        out = np.empty_like(uh)
+        out.real = uh.real * s
+        out.imag = uh.imag * s
+        uh = out
+    return uh
+
+
 DIAG_FIELDS = (
     "step", "time", "dt", "energy", "enstrophy", "helicity", "superhelicity",
     "max_vorticity", "energy_dissipation_rate", "helicity_dissipation_rate",

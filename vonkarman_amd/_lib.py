@@ -115,6 +115,10 @@ SIGNATURES = {
     "vk_solver_physical": (_I, [_P, _P, _P]),
     "vk_solver_device_bytes": (_I, [_P, _PU64]),
     "vk_ic_taylor_green": (_I, [_I, _D, _P]),
+    "vk_ic_abc": (_I, [_I, _D, _D, _D, _D, _P]),
+    "vk_ic_kida_pelz": (_I, [_I, _D, _P]),
+    "vk_ic_anti_parallel_tubes": (_I, [_I, _D, _D, _D, _D, _D, _P]),
+    "vk_ic_random_isotropic_spectrum": (_I, [_I, _D, _D, _D, _U64, _P]),
     "vk_solver_nonlinear_vjp": (_I, [_P, _P, _P, _P]),
     "vk_solver_step_vjp": (_I, [_P, _P, _P]),
     "vk_solver_step_vjp_device": (_I, [_P, _P, _P]),

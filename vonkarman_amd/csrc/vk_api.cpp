@@ -128,6 +128,11 @@ Strided make_strided(int ncols, int hp, int hvalid, int in_inner, long long in_o
 
 }  // namespace
 
+namespace vk {
+// error reporting for the library's other host translation units
+int set_error(int code, const char *msg) { return fail(code, msg); }
+}  // namespace vk
+
 // ===========================================================================
 // FftBackend<f64>
 // ===========================================================================
@@ -1190,30 +1195,6 @@ int vk_solver_adjoint_device(vk_solver *s, int nsteps, int objective, const doub
 int vk_solver_device_bytes(const vk_solver *s, uint64_t *bytes) {
     if (!s || !bytes) return fail(VK_ERR_ARG, "null pointer");
     *bytes = s->total;
-    return VK_OK;
-}
-
-// taylor_green (vonkarman-periodic/src/ic/taylor_green.rs:10-38)
-int vk_ic_taylor_green(int n, double l, double *u) {
-    if (n < 1 || !u) return fail(VK_ERR_ARG, "vk_ic_taylor_green: bad argument");
-    const size_t P = (size_t)n * n * n;
-    const double fn = (double)n;
-    for (int i = 0; i < n; ++i) {
-        const double x = (double)i * l / fn;
-        const double sx = std::sin(x), cx = std::cos(x);
-        for (int j = 0; j < n; ++j) {
-            const double y = (double)j * l / fn;
-            const double sy = std::sin(y), cy = std::cos(y);
-            for (int k = 0; k < n; ++k) {
-                const double z = (double)k * l / fn;
-                const double cz = std::cos(z);
-                const size_t idx = ((size_t)i * n + j) * n + k;
-                u[idx] = sx * cy * cz;
-                u[P + idx] = -(cx * sy * cz);
-                u[2 * P + idx] = 0.0;
-            }
-        }
-    }
     return VK_OK;
 }
 

@@ -1,11 +1,17 @@
 """Initial conditions.
 
-``taylor_green`` restates vonkarman-periodic/src/ic/taylor_green.rs:10-38 (the
-physical field is computed by the C ABI's host routine, glibc sin/cos, exactly
-as the reference evaluates it).  ``von_karman_isotropic`` and
-``antiparallel_tubes`` are the workload initial fields of this backend's
-benchmark configurations (the reference ships neither; its closest are
-ic/random_isotropic.rs and ic/anti_parallel.rs).
+The reference's five (vonkarman-periodic/src/ic/*.rs): ``taylor_green``,
+``abc_flow``, ``kida_pelz`` and ``anti_parallel_tubes`` return the physical
+field real[3, n, n, n], computed by the C ABI's host routines with the
+reference's operations in its order (glibc libm, no fused multiply-adds), so
+they equal the reference's fields bit for bit.  ``random_isotropic_spectrum``
+is random_isotropic.rs up to its final C2R (same generator, draw order, Leray
+projection and energy scaling); ``random_isotropic`` applies that C2R on the
+device.
+
+``von_karman_isotropic`` and ``lamb_oseen_tubes`` are the initial fields of
+this backend's benchmark configurations (a von Karman spectrum with PCG64
+phases, and tubes defined by their vorticity); the reference ships neither.
 """
 from __future__ import annotations
 
@@ -20,6 +26,55 @@ def taylor_green(n: int, l: float = 2.0 * math.pi) -> np.ndarray:
     """u = (sin x cos y cos z, -cos x sin y cos z, 0) on the n^3 grid x_i = i l/n."""
     out = np.empty((3, n, n, n), np.float64)
     check(lib().vk_ic_taylor_green(n, l, ptr(out)), "vk_ic_taylor_green")
+    return out
+
+
+def abc_flow(n: int, a: float = 1.0, b: float = 1.0, c: float = 1.0, l: float = 2.0 * math.pi) -> np.ndarray:
+    """u = (a sin z + c cos y, b sin x + a cos z, c sin y + b cos x)  (abc.rs:14-42)."""
+    out = np.empty((3, n, n, n), np.float64)
+    check(lib().vk_ic_abc(n, l, a, b, c, ptr(out)), "vk_ic_abc")
+    return out
+
+
+def kida_pelz(n: int, l: float = 2.0 * math.pi) -> np.ndarray:
+    """u = (sin x (cos 3y cos z - cos y cos 3z), cyclic)  (kida_pelz.rs:15-48)."""
+    out = np.empty((3, n, n, n), np.float64)
+    check(lib().vk_ic_kida_pelz(n, l, ptr(out)), "vk_ic_kida_pelz")
+    return out
+
+
+def anti_parallel_tubes(n: int, circulation: float = 1.0, core_radius: float = 0.3, separation: float = 1.0,
+                        perturbation: float = 0.1, l: float = 2.0 * math.pi) -> np.ndarray:
+    """Two straight Lamb-Oseen velocity profiles along x with circulation
+    +-circulation, centres (l/2 +- d(x), l/2 -+ separation/2), d(x) =
+    perturbation sin(2 pi x / l)  (anti_parallel.rs:18-104)."""
+    out = np.empty((3, n, n, n), np.float64)
+    check(lib().vk_ic_anti_parallel_tubes(n, l, circulation, core_radius, separation, perturbation, ptr(out)),
+          "vk_ic_anti_parallel_tubes")
+    return out
+
+
+def random_isotropic_spectrum(n: int, k_peak: float = 4.0, energy: float = 0.5, seed: int = 42,
+                              l: float = 2.0 * math.pi) -> np.ndarray:
+    """random_isotropic.rs:18-104: the spectrum before its C2R, complex128[3, n, n, n//2+1].
+    Not Hermitian on the kz = 0 and kz = n/2 planes (the C2R defines the field)."""
+    out = np.empty((3, n, n, n // 2 + 1), np.complex128)
+    check(lib().vk_ic_random_isotropic_spectrum(n, l, k_peak, energy, int(seed) & (2**64 - 1), ptr(out)),
+          "vk_ic_random_isotropic_spectrum")
+    return out
+
+
+def random_isotropic(n: int, k_peak: float = 4.0, energy: float = 0.5, seed: int = 42,
+                     l: float = 2.0 * math.pi) -> np.ndarray:
+    """random_isotropic.rs:18-111: the physical field, the device C2R (1/N^3
+    normalised) of ``random_isotropic_spectrum``.  Needs the GPU."""
+    from .fft import HipFftBackend
+
+    uh = random_isotropic_spectrum(n, k_peak, energy, seed, l)
+    fft = HipFftBackend(n, n, n)
+    out = np.empty((3, n, n, n), np.float64)
+    for c in range(3):
+        fft.c2r_3d(uh[c], out[c])
     return out
 
 
@@ -89,8 +144,8 @@ def von_karman_isotropic(n: int, k0: float = 4.0, kd: float = 64.0, energy: floa
     return np.ascontiguousarray(uh)
 
 
-def antiparallel_tubes(n: int, core_radius: float = 0.3, circulation: float = 1.0,
-                       offset: float = 0.8, amplitude: float = 0.1) -> np.ndarray:
+def lamb_oseen_tubes(n: int, core_radius: float = 0.3, circulation: float = 1.0,
+                     offset: float = 0.8, amplitude: float = 0.1) -> np.ndarray:
     """Two anti-parallel Lamb-Oseen vortex tubes along x (Gaussian vorticity
     profile, circulation +-Gamma, core radius a) centred at y = pi +- offset,
     z = pi, with the centreline displaced by amplitude*cos(x) in y.  The

@@ -39,6 +39,29 @@ class IcType:
         pass
 
     @dataclass(frozen=True)
+    class Abc:
+        a: float = 1.0
+        b: float = 1.0
+        c: float = 1.0
+
+    @dataclass(frozen=True)
+    class AntiParallelTubes:
+        circulation: float = 1.0
+        core_radius: float = 0.3
+        separation: float = 1.0
+        perturbation: float = 0.1
+
+    @dataclass(frozen=True)
+    class KidaPelz:
+        pass
+
+    @dataclass(frozen=True)
+    class RandomIsotropic:
+        k_peak: float = 4.0
+        energy: float = 0.5
+        seed: int = 42
+
+    @dataclass(frozen=True)
     class VonKarmanIsotropic:
         k0: float = 4.0
         kd: float = 64.0
@@ -46,7 +69,11 @@ class IcType:
         seed: int = 42
 
     @dataclass(frozen=True)
-    class AntiParallelTubes:
+    class LambOseenTubes:
+        """Tubes defined by a Gaussian vorticity profile (with periodic images),
+        centred at y = pi +- offset, z = pi, centreline displaced by
+        amplitude*cos(x) in y."""
+
         core_radius: float = 0.3
         circulation: float = 1.0
         offset: float = 0.8
@@ -122,8 +149,20 @@ class Periodic3D:
 
     def _ic_spectral(self, ic) -> np.ndarray:
         n, l = self.n, self._grid.lx
+        # the reference's ICs are physical fields, transformed by r2c_3d (solver.rs:46-76)
         if isinstance(ic, IcType.TaylorGreen):
             u = _ic.taylor_green(n, l)
+        elif isinstance(ic, IcType.Abc):
+            u = _ic.abc_flow(n, ic.a, ic.b, ic.c, l)
+        elif isinstance(ic, IcType.KidaPelz):
+            u = _ic.kida_pelz(n, l)
+        elif isinstance(ic, IcType.AntiParallelTubes):
+            u = _ic.anti_parallel_tubes(n, ic.circulation, ic.core_radius, ic.separation, ic.perturbation, l)
+        elif isinstance(ic, IcType.RandomIsotropic):
+            u = _ic.random_isotropic(n, ic.k_peak, ic.energy, ic.seed, l)
+        else:
+            u = None
+        if u is not None:
             fft = HipFftBackend(n, n, n)
             uh = np.empty((3, n, n, n // 2 + 1), np.complex128)
             for c in range(3):
@@ -131,8 +170,8 @@ class Periodic3D:
             return uh
         if isinstance(ic, IcType.VonKarmanIsotropic):
             return _ic.von_karman_isotropic(n, ic.k0, ic.kd, ic.energy, ic.seed)
-        if isinstance(ic, IcType.AntiParallelTubes):
-            return _ic.antiparallel_tubes(n, ic.core_radius, ic.circulation, ic.offset, ic.amplitude)
+        if isinstance(ic, IcType.LambOseenTubes):
+            return _ic.lamb_oseen_tubes(n, ic.core_radius, ic.circulation, ic.offset, ic.amplitude)
         if isinstance(ic, IcType.Spectral):
             return ic.u_hat
         raise TypeError(f"unknown initial condition {ic!r}")
