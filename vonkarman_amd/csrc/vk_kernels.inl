@@ -890,15 +890,30 @@ __global__ __launch_bounds__((FwdGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
 // Outputs (5 fields, the x pass layout of the next nonlinear evaluation):
 // 0 ux, 1 uy, 2 uz, 3 i kx uz, 4 i kx uy.
 // ===========================================================================
+// The fused pass's plan per stage.  FP32 stage 3 (three state arrays per kept
+// mode) runs the 12-element plan: with the 24-element one its registers spill
+// into AGPRs (256 VGPRs + 54 AGPRs, one wave per SIMD); with 12 elements it
+// needs 228 VGPRs and runs 8-wave blocks at two waves per SIMD.
+#ifndef VK_F32_ST3_E12
+#define VK_F32_ST3_E12 1
+#endif
+template <int L, typename TW, int STG>
+struct FusedPlan {
+    using type = std::conditional_t<sizeof(TW) == 4 && VK_F32_ST3_E12 && STG == ST_3, PlanE12<L>,
+                                    typename StagePlan<L, TW>::type>;
+};
+template <int L, int STG, typename TW>
+using FusedGeom = Geom<L, FusedPlan<L, TW, STG>::type::E, TW, sizeof(TW) == 8 ? VK_ROW_BYTES_FWD : VK_ROW_BYTES_FWD_F32>;
+
 template <int L, int LOUT, int STG, typename TW>
-__global__ __launch_bounds__((FwdGeom<L, FWD_STAGE, TW>::NT)) __attribute__((amdgpu_waves_per_eu(Waves<K_FWD, TW>::value))) void k_fwd_stage_inv(
+__global__ __launch_bounds__((FusedGeom<L, STG, TW>::NT)) __attribute__((amdgpu_waves_per_eu(Waves<K_FWD, TW>::value))) void k_fwd_stage_inv(
     const cpx<TW> *__restrict__ in, cpx<TW> *__restrict__ aout, const cpx<TW> *__restrict__ tw, Strided g, Strided gi,
     int n, double ks, StageArgs sa) {
-    using P = typename StagePlan<L, TW>::type;
+    using P = typename FusedPlan<L, TW, STG>::type;
     using PL = typename P::radices;         // forward order
     using RL = typename Reverse<PL>::type;  // inverse order, first radix = forward's last
     constexpr int E = P::E;
-    using G = FwdGeom<L, FWD_STAGE, TW>;
+    using G = FusedGeom<L, STG, TW>;
     constexpr int T = G::T, NT = G::NT;
     constexpr int RP = PL::last;
     constexpr int KR = 2 * RP / 3, K = (E / RP) * KR;
@@ -1018,7 +1033,7 @@ __global__ __launch_bounds__((FwdGeom<L, FWD_STAGE, TW>::NT)) __attribute__((amd
 template <int L, int LOUT, int STG, typename TW>
 int run_fwd_inv(const void *in, void *aout, const void *tw, const Strided &g, const Strided &gi, int n, double ks,
                 const StageArgs *sa, hipStream_t st) {
-    using C = FwdGeom<L, FWD_STAGE, TW>;
+    using C = FusedGeom<L, STG, TW>;
     if (pad_stores<TW, C::T>() && g.ncols % C::T != 0) return 2;
     const unsigned nb = (unsigned)((g.ncols + C::T - 1) / C::T);
     hipLaunchKernelGGL((k_fwd_stage_inv<L, LOUT, STG, TW>), dim3(nb), dim3(C::NT), 0, st,
