@@ -147,11 +147,13 @@ struct vk_fft_plan {
     std::string name;
 };
 
-static int plan_c2r_kernels(vk_fft_plan *p, const void *spec_padded) {
-    // x inverse: t1 natural [ix][iy][hp] -> t2 [iy][ix][hp]
+static int plan_c2r_kernels(vk_fft_plan *p, const void *spec) {
+    // x inverse: the caller's spectrum [ix][iy][hz] (rows of hz, unpadded) ->
+    // t2 [iy][ix][hp].  Column (iy, iz) of the padded column space reads
+    // iy*hz + iz; padding columns (iz >= hz) read nothing.
     const int nx = p->nx, ny = p->ny, nz = p->nz, hp = p->hp, hz = p->hz;
-    Strided gx = make_strided(ny * hp, hp, hz, ny * hp, 0, (long long)ny * hp, 0, hp, (long long)nx * hp, hp, 0);
-    VK_TRY(launch_inv(INV_PLAIN, nx, nx, F64, F64, spec_padded, p->t2.p, p->tw_x.p, gx, nx, 1.0, 1.0, p->st));
+    Strided gx = make_strided(ny * hp, hp, hz, hp, hz, (long long)ny * hz, 0, hp, (long long)nx * hp, hp, 0);
+    VK_TRY(launch_inv(INV_PLAIN, nx, nx, F64, F64, spec, p->t2.p, p->tw_x.p, gx, nx, 1.0, 1.0, p->st));
     // y inverse: t2 [iy][ix][hp] -> t1 [ix][iy][hp]
     Strided gy = make_strided(nx * hp, hp, hz, nx * hp, 0, (long long)nx * hp, 0, hp, (long long)ny * hp, hp, 0);
     VK_TRY(launch_inv(INV_PLAIN, ny, ny, F64, F64, p->t2.p, p->t1.p, p->tw_y.p, gy, ny, 1.0, 1.0, p->st));
@@ -257,8 +259,7 @@ static int fft_c2r_impl(vk_fft_plan *p, const double *in, double *out, bool devi
     VK_HIP(hipEventRecord(p->ev0, p->st));
     VK_HIP(hipMemcpyAsync(p->spec.p, in, spec_bytes, device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, p->st));
     VK_HIP(hipEventRecord(p->ev1, p->st));
-    VK_TRY(launch_repack(p->spec.p, p->t1.p, (long long)p->nx * p->ny, p->hz, p->hp, p->hz, 1, p->st));
-    if (int r = plan_c2r_kernels(p, p->t1.p)) return r;
+    if (int r = plan_c2r_kernels(p, p->spec.p)) return r;
     VK_HIP(hipEventRecord(p->ev2, p->st));
     VK_HIP(hipMemcpyAsync(out, p->real.p, real_bytes, device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, p->st));
     VK_HIP(hipEventRecord(p->ev3, p->st));
