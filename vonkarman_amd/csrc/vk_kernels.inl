@@ -29,7 +29,10 @@ enum KernelClass { K_INV, K_FWD, K_FVJP, K_ZNL, K_ZVJP };
 #define VK_WAVES_FWD_F32 1
 #endif
 #ifndef VK_WAVES_ZNL_F32
-#define VK_WAVES_ZNL_F32 4  // 128 VGPRs: 4 blocks of 4 waves per CU (1.47x faster than uncapped)
+// FP32 z pass: a floor of 3 waves per SIMD.  The compiler then fits the
+// pipelined loads in 128 VGPRs without scratch, which is 4 waves per SIMD
+// (a floor of 4 spills 16 B per lane; no floor takes ~200 VGPRs, 1.47x slower)
+#define VK_WAVES_ZNL_F32 3
 #endif
 #ifndef VK_WAVES_ZNL_F64
 #define VK_WAVES_ZNL_F64 1
@@ -1177,13 +1180,13 @@ __device__ __forceinline__ void z_pair_fill(cpx<TW> (&v)[E], gptr<cpx<TW>> fa, g
 #define VK_Z_LDS_TW 1
 #endif
 // the nonlinear z pass issues each next input pair's loads before the current
-// pair's transform (FP64: 0.738 -> 0.704 ms at 384).  Not in FP32, where the
-// raw loads push the kernel past its 128-VGPR cap (0.387 -> 0.398 ms).
+// pair's transform (at 384: FP64 0.738 -> 0.704 ms; FP32 0.395 -> 0.387 ms,
+// at 128 VGPRs under its 3-wave floor)
 #ifndef VK_Z_PIPE_F64
 #define VK_Z_PIPE_F64 1
 #endif
 #ifndef VK_Z_PIPE_F32
-#define VK_Z_PIPE_F32 0
+#define VK_Z_PIPE_F32 1
 #endif
 template <int L, int D, int T, int NT, typename PL, int E, typename R>
 __device__ __forceinline__ void zfft(cpx<R> (&v)[E], cpx<R> *lds, const cpx<R> *ltw, const cpx<R> *tw, int tid) {
