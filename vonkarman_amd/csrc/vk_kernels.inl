@@ -922,7 +922,12 @@ __global__ __launch_bounds__((FusedGeom<L, STG, TW>::NT)) __attribute__((amdgpu_
     constexpr int KR = 2 * RP / 3, K = (E / RP) * KR;
     static_assert(RP % 3 == 0 && 3 * LOUT == 2 * L, "stage pass needs a 3/2-padded plan");
     constexpr bool HEAVY = STG == ST_3;
-    constexpr int CH = sizeof(TW) == 8 ? (HEAVY ? 1 : 2) : (HEAVY ? 2 : 4);
+    // kept modes per epilogue chunk.  FP64 takes one: two spilled 40 B per
+    // lane in stages 1-2 (fused pass 0.663 -> 0.648 ms with one, same box).
+#ifndef VK_FUSED_CH_F64
+#define VK_FUSED_CH_F64 1
+#endif
+    constexpr int CH = sizeof(TW) == 8 ? (HEAVY ? 1 : VK_FUSED_CH_F64) : (HEAVY ? 2 : 4);
     static_assert(K % CH == 0, "chunks must tile the kept modes");
     __shared__ cpx<TW> lds[L * T];
     __shared__ cpx<TW> ltw[L];
