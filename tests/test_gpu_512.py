@@ -1,5 +1,5 @@
-"""Parity at 512^3 (768^3 padded: the 768-point plans of every pass) through
-band-limiting.  A field whose modes satisfy |k_i| <= 3 stays exactly
+"""Parity at 512^3 (768^3 padded: the 768-point plans of every pass and of the
+adjoint) through band-limiting.  A field whose modes satisfy |k_i| <= 3 stays exactly
 representable through one ETD-RK4 step on any grid with N >= 128. Its stage
 inputs reach |k_i| <= 24 and their products |k_i| <= 48, all kept and
 alias-free under the 3/2 rule at N = 128.  So the 512^3 step of the embedded
@@ -65,4 +65,37 @@ def test_step_at_512_matches_band_limited_oracle(hip_device, prec, tol):
     ref = _embed(ref_s, NL)
     err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
     print(f"  {prec}: rel L2 vs embedded oracle {err:.2e}", flush=True)
+    assert err < tol
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("mixed", 1e-5)])
+def test_step_vjp_at_512_matches_band_limited_oracle(hip_device, prec, tol):
+    """The discrete adjoint at 512^3 (the 768-point VJP kernels).  With u and the
+    cotangent both band-limited to |k_i| <= 3, every product in the step's VJP
+    stays within |k_i| <= 48, so the VJP is the 128^3 one (in mode coordinates
+    the step's Jacobian does not depend on the grid)."""
+    from vonkarman_amd import GridSpec, IcType, Periodic3D
+
+    print(f"case 512^3 band-limited step VJP {prec}", flush=True)
+    nu, dt = 1e-3, 2e-3
+    us = _band_limited(NS, KB, 11)
+    rng = np.random.default_rng(4)
+    lam = np.zeros_like(us)
+    idx = [k % NS for k in range(-KB, KB + 1)]
+    lam[np.ix_(range(3), idx, idx, range(KB + 1))] = (rng.standard_normal((3, 2 * KB + 1, 2 * KB + 1, KB + 1))
+                                                      + 1j * rng.standard_normal((3, 2 * KB + 1, 2 * KB + 1, KB + 1)))
+    r = O.Solver(NS, nu)
+    r.set_state(us)
+    r.set_dt(dt, True)
+    ref_s = r.step_vjp(lam)
+    del r
+    assert np.abs(ref_s[:, NS // 2 - 1:NS // 2 + 2]).max() < 1e-14 * np.abs(ref_s).max()
+    unscale = (NS / NL) ** 3  # _embed scales by (NL/NS)^3; the cotangent and the VJP are not scaled
+    s = Periodic3D(GridSpec.cubic(NL), nu, IcType.Spectral(_embed(us, NL)), precision=prec, init_dt=False)
+    s.set_fixed_dt(dt)
+    got = s.step_vjp(_embed(lam, NL) * unscale)
+    del s
+    ref = _embed(ref_s, NL) * unscale
+    err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+    print(f"  {prec}: rel L2 vs embedded oracle VJP {err:.2e}", flush=True)
     assert err < tol
