@@ -68,15 +68,16 @@ def test_step_at_512_matches_band_limited_oracle(hip_device, prec, tol):
     assert err < tol
 
 
+@pytest.mark.parametrize("nl", [256, NL])
 @pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("mixed", 1e-5)])
-def test_step_vjp_at_512_matches_band_limited_oracle(hip_device, prec, tol):
-    """The discrete adjoint at 512^3 (the 768-point VJP kernels).  With u and the
+def test_step_vjp_at_512_matches_band_limited_oracle(hip_device, prec, tol, nl):
+    """The discrete adjoint at 256^3 and 512^3 (the 384- and 768-point VJP kernels).  With u and the
     cotangent both band-limited to |k_i| <= 3, every product in the step's VJP
     stays within |k_i| <= 48, so the VJP is the 128^3 one (in mode coordinates
     the step's Jacobian does not depend on the grid)."""
     from vonkarman_amd import GridSpec, IcType, Periodic3D
 
-    print(f"case 512^3 band-limited step VJP {prec}", flush=True)
+    print(f"case {nl}^3 band-limited step VJP {prec}", flush=True)
     nu, dt = 1e-3, 2e-3
     us = _band_limited(NS, KB, 11)
     rng = np.random.default_rng(4)
@@ -90,12 +91,12 @@ def test_step_vjp_at_512_matches_band_limited_oracle(hip_device, prec, tol):
     ref_s = r.step_vjp(lam)
     del r
     assert np.abs(ref_s[:, NS // 2 - 1:NS // 2 + 2]).max() < 1e-14 * np.abs(ref_s).max()
-    unscale = (NS / NL) ** 3  # _embed scales by (NL/NS)^3; the cotangent and the VJP are not scaled
-    s = Periodic3D(GridSpec.cubic(NL), nu, IcType.Spectral(_embed(us, NL)), precision=prec, init_dt=False)
+    unscale = (NS / nl) ** 3  # _embed scales by (nl/NS)^3; the cotangent and the VJP are not scaled
+    s = Periodic3D(GridSpec.cubic(nl), nu, IcType.Spectral(_embed(us, nl)), precision=prec, init_dt=False)
     s.set_fixed_dt(dt)
-    got = s.step_vjp(_embed(lam, NL) * unscale)
+    got = s.step_vjp(_embed(lam, nl) * unscale)
     del s
-    ref = _embed(ref_s, NL) * unscale
+    ref = _embed(ref_s, nl) * unscale
     err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
     print(f"  {prec}: rel L2 vs embedded oracle VJP {err:.2e}", flush=True)
     assert err < tol
