@@ -100,3 +100,23 @@ def test_step_vjp_at_512_matches_band_limited_oracle(hip_device, prec, tol, nl):
     err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
     print(f"  {prec}: rel L2 vs embedded oracle VJP {err:.2e}", flush=True)
     assert err < tol
+
+
+def test_rk4_step_at_512_matches_band_limited_oracle(hip_device):
+    """The RK4 cross-check integrator at 512^3 (rk4.rs:16-146): its stage inputs
+    reach |k_i| <= 24 as in ETD-RK4, so one step equals the oracle's 128^3 step."""
+    from vonkarman_amd import GridSpec, IcType, Periodic3D
+
+    print("case 512^3 band-limited rk4 step", flush=True)
+    nu, dt = 1e-3, 2e-3
+    us = _band_limited(NS, KB, 12)
+    ref_s = O.rk4_step(us, nu, dt, nsteps=1)
+    assert np.abs(ref_s[:, NS // 2 - 1:NS // 2 + 2]).max() < 1e-14 * np.abs(ref_s).max()
+    s = Periodic3D(GridSpec.cubic(NL), nu, IcType.Spectral(_embed(us, NL)), init_dt=False)
+    s.rk4_step(dt, 1)
+    got = s.u_hat()
+    del s
+    ref = _embed(ref_s, NL)
+    err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+    print(f"  rel L2 vs embedded oracle {err:.2e}", flush=True)
+    assert err < 1e-12
