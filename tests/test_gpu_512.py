@@ -54,6 +54,7 @@ def test_step_at_512_matches_band_limited_oracle(hip_device, prec, tol):
     r.set_dt(dt, True)
     r.step(1)
     ref_s = r.state()
+    od, oek = r.diagnostics(), r.energy_spectrum()
     del r
     # the band the 128^3 step fills stays inside the kept modes of both grids
     assert np.abs(ref_s[:, NS // 2 - 1:NS // 2 + 2]).max() < 1e-14 * np.abs(ref_s).max()
@@ -61,6 +62,17 @@ def test_step_at_512_matches_band_limited_oracle(hip_device, prec, tol):
     s.set_fixed_dt(dt)
     s.steps(1)
     got = s.u_hat()
+    # the normalised diagnostics and the shell spectrum do not depend on the
+    # grid for a band-limited field (max|w| does: it samples physical space)
+    if prec == "f64":
+        from vonkarman_amd import ScalarDiagnostics
+
+        d = ScalarDiagnostics.from_domain(s)
+        for k in ("energy", "enstrophy", "helicity", "superhelicity", "energy_dissipation_rate"):
+            assert abs(getattr(d, k) - od[k]) <= 1e-12 * max(abs(od[k]), 1e-300), (k, getattr(d, k), od[k])
+        ek = s.energy_spectrum_array()
+        assert np.abs(ek[:len(oek)] - oek).max() <= 1e-12 * oek.max()
+        assert np.abs(ek[len(oek):]).max() <= 1e-14 * oek.max()
     del s
     ref = _embed(ref_s, NL)
     err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
