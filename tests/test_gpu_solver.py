@@ -67,7 +67,7 @@ def test_one_fixed_step_random(hip_device, n):
 
 
 def test_taylor_green_100_steps_fp64(hip_device):
-    """TG N=64, nu=6.25e-4, fixed dt=1e-3, 100 steps; spectral
+    """Taylor-Green at N=64 (Re 1600), nu=6.25e-4, fixed dt=1e-3, 100 steps; spectral
     coefficients within relative L2 1e-12 of the oracle; diagnostics and E(k)."""
     from vonkarman_amd import GridSpec, IcType, Periodic3D, ScalarDiagnostics
 
