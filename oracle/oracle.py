@@ -7,6 +7,12 @@ benchmark import this module; the product package never does.
 Arrays follow the reference's layout: real fields ``float64[nx, ny, nz]``,
 spectral fields ``complex128[nx, ny, nz//2+1]``, vector fields stacked on a
 leading axis of length 3.
+
+The reference's initial conditions other than Taylor-Green are restated here
+in Python/numpy (small grids only).  tests/test_ic.py pins them by closed
+forms: the ABC energy and curl u = u, the Kida-Pelz energy and symmetry, the
+Lamb-Oseen profile in polar form, the xorshift64 generator's published first
+output and the amplitude law of the random spectrum.
 """
 from __future__ import annotations
 
