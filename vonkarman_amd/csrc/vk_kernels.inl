@@ -40,8 +40,11 @@ enum KernelClass { K_INV, K_FWD, K_FVJP, K_ZNL, K_ZVJP };
 #ifndef VK_WAVES_INV_F64
 #define VK_WAVES_INV_F64 1
 #endif
+// FP64 z VJP pass: a 2-wave floor.  Unconstrained it takes 256 VGPRs + 38
+// AGPRs (one wave per SIMD); at two waves it spills 172 B per lane to scratch
+// and still runs the reverse step 24.3 -> 22.9 ms at 256^3.
 #ifndef VK_WAVES_ZVJP_F64
-#define VK_WAVES_ZVJP_F64 1
+#define VK_WAVES_ZVJP_F64 2
 #endif
 #ifndef VK_WAVES_FWD_F64
 #define VK_WAVES_FWD_F64 2  // 256 VGPRs: keeps the stage-3/4 epilogues at 2 waves (no AGPR overflow)
