@@ -46,6 +46,11 @@ enum KernelClass { K_INV, K_FWD, K_FVJP, K_ZNL, K_ZVJP };
 #ifndef VK_WAVES_ZVJP_F64
 #define VK_WAVES_ZVJP_F64 2
 #endif
+// FP64 x/y VJP passes: no floor (256 VGPRs + up to 176 AGPRs, one wave per
+// SIMD).  A 2-wave floor spills ~600 B per lane: reverse step 22.8 -> 23.2 ms.
+#ifndef VK_WAVES_FVJP_F64
+#define VK_WAVES_FVJP_F64 1
+#endif
 #ifndef VK_WAVES_FWD_F64
 #define VK_WAVES_FWD_F64 2  // 256 VGPRs: keeps the stage-3/4 epilogues at 2 waves (no AGPR overflow)
 #endif
@@ -55,6 +60,7 @@ struct Waves {
                                                     : K == K_INV  ? VK_WAVES_INV_F64
                                                     : K == K_FWD  ? VK_WAVES_FWD_F64
                                                     : K == K_ZVJP ? VK_WAVES_ZVJP_F64
+                                                    : K == K_FVJP ? VK_WAVES_FVJP_F64
                                                                   : 1)
                                  : K == K_INV  ? VK_WAVES_INV_F32
                                  : K == K_FWD  ? VK_WAVES_FWD_F32
