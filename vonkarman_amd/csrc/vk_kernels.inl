@@ -48,6 +48,11 @@ enum KernelClass { K_INV, K_FWD, K_FVJP, K_ZNL, K_ZVJP };
 #endif
 // FP64 x/y VJP passes: no floor (256 VGPRs + up to 176 AGPRs, one wave per
 // SIMD).  A 2-wave floor spills ~600 B per lane: reverse step 22.8 -> 23.2 ms.
+// FP32 z VJP pass: no floor (174 VGPRs, two waves per SIMD); a 3-wave floor
+// (168 VGPRs, 44 B spilled per lane) measured the same.
+#ifndef VK_WAVES_ZVJP_F32
+#define VK_WAVES_ZVJP_F32 1
+#endif
 #ifndef VK_WAVES_FVJP_F64
 #define VK_WAVES_FVJP_F64 1
 #endif
@@ -65,6 +70,7 @@ struct Waves {
                                  : K == K_INV  ? VK_WAVES_INV_F32
                                  : K == K_FWD  ? VK_WAVES_FWD_F32
                                  : K == K_ZNL  ? VK_WAVES_ZNL_F32
+                                 : K == K_ZVJP ? VK_WAVES_ZVJP_F32
                                                : 1;
 };
 
