@@ -218,6 +218,15 @@ int vk_ic_anti_parallel_tubes(int n, double l, double circulation, double core_r
  * is vk_fft_c2r_3d of each component (random_isotropic.rs:105-110).  n even. */
 int vk_ic_random_isotropic_spectrum(int n, double l, double k_peak, double total_energy, uint64_t seed,
                                     double *u_hat);
+/* No reference counterpart: the von Karman isotropic field of the benchmark
+ * configurations, complex[3][n][n][n/2+1] (interleaved re, im).  Amplitude
+ * sqrt(E(|k|) / (4 pi |k|^2)) per mode and component with
+ * E(k) = (k/k0)^4 / (1+(k/k0)^2)^(17/6) exp(-2 (k/kd)^2), phases from
+ * xorshift64 in random_isotropic.rs:30-38's generator and draw order, Leray
+ * projection, Hermitian kz = 0 and kz = n/2 planes, scaled to total_energy.
+ * n even, seed != 0. */
+int vk_ic_von_karman_spectrum(int n, double l, double k0, double kd, double total_energy, uint64_t seed,
+                              double *u_hat);
 
 #ifdef __cplusplus
 }

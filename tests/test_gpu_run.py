@@ -80,3 +80,36 @@ def test_run_loop_rows_match_oracle_and_restart_is_bitwise(hip_device, tmp_path,
     b6 = read_checkpoint(s_b.checkpoints[-1])
     assert b6.step_count == 6 and b6.time == a6.time and b6.dt == a6.dt
     assert np.array_equal(a6.u_hat, b6.u_hat)
+
+
+def test_command_line_run_and_restart(hip_device, tmp_path):
+    """`python -m vonkarman_amd.run run --config FILE --backend hip` (main.rs:
+    19-66) in a child process writes the run's Parquet rows and checkpoints;
+    `--restart` continues from the step-3 checkpoint to the same final state."""
+    import os
+    import subprocess
+    import sys
+
+    import pyarrow.parquet as pq
+
+    from vonkarman_amd import read_checkpoint
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    print("case command line run n=16", flush=True)
+    cfg_a = tmp_path / "a.toml"
+    cfg_a.write_text(CONFIG.format(out=tmp_path / "a", ic="taylor-green", tier="f64").replace('backend = "hip"\n', ""))
+    cmd = [sys.executable, "-m", "vonkarman_amd.run", "run", "--config", str(cfg_a), "--backend", "hip"]
+    r = subprocess.run(cmd, cwd=root, capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "simulation complete" in r.stderr
+    rows = pq.read_table(str(tmp_path / "a" / "diagnostics.parquet")).to_pydict()
+    assert rows["step"] == [0, 2, 4, 6]
+    cfg_b = tmp_path / "b.toml"
+    cfg_b.write_text(CONFIG.format(out=tmp_path / "b", ic="taylor-green", tier="f64"))
+    ck3 = str(tmp_path / "a" / "checkpoint_00000003.npz")
+    r = subprocess.run([sys.executable, "-m", "vonkarman_amd.run", "run", "--config", str(cfg_b), "--restart", ck3],
+                       cwd=root, capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr[-2000:]
+    a6 = read_checkpoint(str(tmp_path / "a" / "checkpoint_00000006.npz"))
+    b6 = read_checkpoint(str(tmp_path / "b" / "checkpoint_00000006.npz"))
+    assert b6.step_count == 6 and np.array_equal(a6.u_hat, b6.u_hat)

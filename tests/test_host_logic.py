@@ -75,6 +75,10 @@ def _div(uh):
 
 
 def test_von_karman_ic_properties():
+    """The benchmark field: energy 0.5, Hermitian kz = 0 and kz = n/2 planes
+    (so the C2R/R2C round trip returns it), divergence-free wherever the
+    symmetrisation pairs k with -k, seeded determinism, and the shell spectrum
+    of the von Karman law up to sampling noise."""
     from vonkarman_amd import ic
 
     n = 32
@@ -82,16 +86,22 @@ def test_von_karman_ic_properties():
     assert uh.shape == (3, n, n, n // 2 + 1)
     s = O.Solver(n, 1e-3)
     s.set_state(uh)
-    assert abs(s.energy() - 0.5) < 1e-13
-    assert np.abs(_div(uh)).max() < 1e-10 * np.abs(uh).max()
-    # Hermitian kz=0 plane -> the C2R of the field is the real field it encodes
+    assert abs(s.energy() - 0.5) < 1e-14
+    # modes whose Hermitian partner is -k: interior kz, and kz = 0 off the
+    # Nyquist rows (index n/2 is k = -n/2 for the mode and its partner alike)
+    d = _div(uh)
+    keep = np.ones(d.shape, bool)
+    keep[:, :, n // 2] = False
+    keep[n // 2, :, 0] = False
+    keep[:, n // 2, 0] = False
+    assert np.abs(d[keep]).max() < 1e-12 * np.abs(uh).max()
     neg = (-np.arange(n)) % n
-    p = uh[:, :, :, 0]
-    assert np.abs(p - np.conj(p[:, neg][:, :, neg])).max() < 1e-12 * np.abs(p).max()
+    for iz in (0, n // 2):
+        p = uh[:, :, :, iz]
+        assert np.array_equal(p, np.conj(p[:, neg][:, :, neg]))
     phys = np.stack([O.c2r_3d(uh[c]) for c in range(3)])
     back = np.stack([O.r2c_3d(phys[c]) for c in range(3)])
     assert np.abs(back - uh).max() < 1e-10 * np.abs(uh).max()
-    # deterministic in the seed, different across seeds
     assert np.array_equal(uh, ic.von_karman_isotropic(n, seed=42))
     assert not np.array_equal(uh, ic.von_karman_isotropic(n, seed=43))
     # the shell spectrum follows E(k) = (k/k0)^4 / (1+(k/k0)^2)^(17/6) exp(-2 (k/kd)^2)
@@ -101,6 +111,76 @@ def test_von_karman_ic_properties():
     law = (k / 4) ** 4 / (1 + (k / 4) ** 2) ** (17 / 6) * np.exp(-2 * (k / 64) ** 2)
     ratio = e[3:15] / law
     assert ratio.max() / ratio.min() < 1.4
+
+
+def test_von_karman_ic_generator_pins():
+    """The first mode drawn is k = (0, 0, 1): its x and y components keep the
+    phases 2 pi r1 and 2 pi r2 of the first two xorshift64 draws (the Leray
+    projection along z leaves them and the energy scale is real), and the
+    amplitudes of the kz-axis modes follow sqrt(E(k) / (4 pi k^2)) exactly."""
+    from vonkarman_amd import ic
+
+    n, k0, kd = 16, 3.0, 5.0
+    uh = ic.von_karman_isotropic(n, k0=k0, kd=kd, energy=0.7, seed=12345)
+    g = O.xorshift64(12345)
+    r1, r2 = next(g) / float(2 ** 64 - 1), next(g) / float(2 ** 64 - 1)
+    for comp, r in ((0, r1), (1, r2)):
+        want = np.angle(np.exp(2j * np.pi * r))
+        assert abs(np.angle(uh[comp, 0, 0, 1]) - want) < 1e-12
+
+    def amp(k):
+        return math.sqrt((k / k0) ** 4 / (1 + (k / k0) ** 2) ** (17 / 6) * math.exp(-2 * (k / kd) ** 2)
+                         / (4 * math.pi * k * k))
+
+    for kz in (2, 3, 5):
+        got = abs(uh[0, 0, 0, kz]) / abs(uh[0, 0, 0, 1])
+        assert abs(got - amp(kz) / amp(1)) < 1e-13 * got
+    assert abs(uh[2, 0, 0, 1]) == 0.0  # the projection removes u_z along k = z
+
+
+def test_lamb_oseen_vorticity_pins():
+    """The tubes' vorticity: over the strip pi < y < 2 pi (one tube) the
+    integral of omega_x is +Gamma and over 0 < y < pi it is -Gamma, and the
+    vorticity-weighted centre of the upper tube is pi + offset +
+    amplitude cos x (to the Gaussian tails cut by the strip)."""
+    from vonkarman_amd import ic
+
+    n, L = 32, 2 * math.pi
+    y = np.arange(n) * L / n
+    for gam, a, off, amp in ((1.0, 0.3, 0.8, 0.1), (2.0, 0.25, 0.9, -0.15)):
+        wx = ic.lamb_oseen_vorticity(n, core_radius=a, circulation=gam, offset=off, amplitude=amp)
+        up, lo = y > math.pi, (y > 0) & (y < math.pi)
+        for ix in range(0, n, 4):
+            x = ix * L / n
+            f = wx[ix][up]
+            assert abs(f.sum() * (L / n) ** 2 - gam) < 2e-3 * gam
+            assert abs(wx[ix][lo].sum() * (L / n) ** 2 + gam) < 2e-3 * gam
+            cen = (y[up][:, None] * f).sum() / f.sum()
+            assert abs(cen - (math.pi + off + amp * math.cos(x))) < 2e-3
+
+
+def test_lamb_oseen_velocity_from_vorticity():
+    """u_hat = i k x P(omega_hat) / |k|^2: divergence-free, its curl is the
+    projected vorticity (off the zeroed Nyquist modes), and its circulation
+    around the strip of each tube is +-Gamma (Stokes; the line integral of u_z
+    along y = 0 and y = pi over a period of z)."""
+    from vonkarman_amd import ic
+
+    n, L = 32, 2 * math.pi
+    wxh = O.r2c_3d(ic.lamb_oseen_vorticity(n))
+    uh = ic.velocity_from_vorticity(wxh)
+    assert np.abs(_div(uh)).max() < 1e-14 * np.abs(uh).max()
+    wh = np.zeros_like(uh)
+    wh[0] = wxh
+    pw = O.leray(wh)
+    keep = np.ones(uh.shape[1:], bool)
+    keep[n // 2, :, :] = keep[:, n // 2, :] = keep[:, :, n // 2] = False
+    assert np.abs((O.curl(uh) - pw)[:, keep]).max() < 1e-13 * np.abs(pw).max()
+    uz = O.c2r_3d(uh[2])
+    for ix in range(0, n, 4):
+        gam_up = L * (uz[ix, 0, :].mean() - uz[ix, n // 2, :].mean())
+        assert abs(gam_up - 1.0) < 5e-3
+        assert abs(uz[ix, :, :].mean()) < 1e-15  # the two strips cancel
 
 
 def test_gridspec_mirrors_reference():

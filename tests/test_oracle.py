@@ -304,6 +304,108 @@ def test_spectrum_single_mode_and_total():
     assert abs(np.sum(O.energy_spectrum(uh)) - s.energy()) / s.energy() < 1e-10
 
 
+def _spec(u):
+    return np.stack([O.r2c_3d(np.ascontiguousarray(u[c])) for c in range(3)])
+
+
+def beltrami_z(n, k, handed, l=TWO_PI):
+    """u = (sin kz, cos kz, 0) (handed=+1: curl u = +k u) or
+    u = (cos kz, sin kz, 0) (handed=-1: curl u = -k u), with 2 pi / l = 1."""
+    z = np.arange(n) * l / n
+    u = np.zeros((3, n, n, n))
+    a, b = (np.sin(k * z), np.cos(k * z)) if handed > 0 else (np.cos(k * z), np.sin(k * z))
+    u[0] = a[None, None, :]
+    u[1] = b[None, None, :]
+    return u
+
+
+@pytest.mark.parametrize("k,handed", [(1, 1), (1, -1), (2, 1), (3, -1)])
+def test_diagnostics_of_beltrami_mode_closed_forms(k, handed):
+    """For curl u = s k u with <|u|^2> = 1 (s = +-1): E = 1/2, Omega = k^2,
+    H = <u.w> = s k and S = <w.curl w> = s k^3 (solver.rs:412-535).  A wrong
+    sign of H or S, a dropped R2C weight, a missing 1/N^6 or a curl applied
+    the wrong number of times each fails one of these."""
+    n, nu = 16, 0.01
+    s = O.Solver(n, nu)
+    s.set_state(_spec(beltrami_z(n, k, handed)))
+    assert abs(s.energy() - 0.5) < 1e-14
+    assert abs(s.enstrophy() - k ** 2) < 1e-13 * k ** 2
+    assert abs(s.helicity() - handed * k) < 1e-13 * k
+    assert abs(s.superhelicity() - handed * k ** 3) < 1e-13 * k ** 3
+    d = s.diagnostics()
+    assert abs(d["helicity_dissipation_rate"] - (-2 * nu * handed * k ** 3)) < 1e-13 * nu * k ** 3
+    assert abs(d["energy_dissipation_rate"] - (-2 * nu * k ** 2)) < 1e-13 * nu * k ** 2
+
+
+@pytest.mark.parametrize("abc", [(1.0, 1.0, 1.0), (1.0, 0.7, 0.3)])
+def test_diagnostics_of_abc_flow_closed_forms(abc):
+    """ABC flow is Beltrami with curl u = u (abc.rs:14-42), so
+    Omega = H = S = <|u|^2> = a^2 + b^2 + c^2 and E = (a^2 + b^2 + c^2)/2
+    (energy_budget.rs:102-103 uses E0 = 1.5 for a = b = c = 1)."""
+    n, nu = 16, 0.01
+    a, b, c = abc
+    q = a * a + b * b + c * c
+    s = O.Solver(n, nu)
+    s.set_state(_spec(O.abc_flow(n, a, b, c)))
+    assert abs(s.energy() - 0.5 * q) < 1e-13
+    for v in (s.enstrophy(), s.helicity(), s.superhelicity()):
+        assert abs(v - q) < 1e-13 * q
+    d = s.diagnostics()
+    assert abs(d["helicity_dissipation_rate"] - (-2 * nu * q)) < 1e-13
+
+
+def test_taylor_green_spectrum_lies_in_shell_two():
+    """All TG modes have |k| = sqrt(3) = 1.732, and shell = round(|k|)
+    (spectrum.rs:41-42) puts all of E = 1/8 in shell 2 (floor would give 1)."""
+    n = 16
+    e = O.energy_spectrum(tg_state(n))
+    assert abs(e[2] - 0.125) < 1e-15
+    assert np.abs(np.delete(e, 2)).max() < 1e-30
+
+
+def _single_mode(n, ix, iy, iz, amp=1.0):
+    u = np.zeros((3, n, n, n // 2 + 1), complex)
+    u[0, ix, iy, iz] = amp * n ** 3
+    return u
+
+
+@pytest.mark.parametrize("kvec,shell", [
+    ((1, 1, 1), 2),   # |k| = 1.732: round -> 2, floor -> 1
+    ((1, 1, 0), 1),   # 1.414: round -> 1, ceil -> 2
+    ((2, 1, 1), 2),   # 2.449
+    ((3, 1, 0), 3),   # 3.162
+    ((2, 2, 1), 3),   # exactly 3
+    ((1, 2, 2), 3),
+    ((3, 2, 1), 4),   # 3.742
+    ((-4, 0, 0), 4),  # the Nyquist row (index n/2 is k = -n/2)
+])
+def test_shell_index_is_rounded_magnitude(kvec, shell):
+    """spectrum.rs:41-47: shell = round(sqrt(kx^2 + ky^2 + kz^2)) with the
+    wavenumber layout of spectral_ops.rs:55-61.  Interior kz modes carry the
+    R2C weight 2, so E(shell) = |u|^2 / N^6 there and |u|^2 / (2 N^6) at
+    kz = 0."""
+    n = 8
+    kx, ky, kz = kvec
+    ix, iy = kx % n, ky % n
+    e = O.energy_spectrum(_single_mode(n, ix, iy, kz))
+    want = 1.0 if 0 < kz < n // 2 else 0.5
+    assert abs(e[shell] - want) < 1e-15, (kvec, e)
+    assert np.abs(np.delete(e, shell)).max() == 0.0
+
+
+@pytest.mark.parametrize("kx,shell", [(1, 1), (3, 2), (5, 3), (7, 4)])
+def test_shell_index_rounds_half_away_from_zero_at_l_4pi(kx, shell):
+    """With l = 4 pi the wavenumbers are i/2, so |k| = 0.5, 1.5, 2.5, 3.5 are
+    exact ties: Rust's f64::round (spectrum.rs:42) rounds them away from zero
+    (round-half-even would put 0.5 and 2.5 in shells 0 and 2).  The shell
+    count stays ceil(sqrt(3) n/2) + 1, from the index range (spectrum.rs:22-26)."""
+    n = 16
+    e = O.energy_spectrum(_single_mode(n, kx, 0, 0), l=2 * TWO_PI)
+    assert e.size == math.ceil(math.sqrt(3) * n / 2) + 1
+    assert abs(e[shell] - 0.5) < 1e-15
+    assert np.abs(np.delete(e, shell)).max() == 0.0
+
+
 def test_energy_decays_monotonically_tg16():
     """solver.rs:625-650 (adaptive dt, 50 steps)."""
     n, nu = 16, 0.01

@@ -119,6 +119,7 @@ SIGNATURES = {
     "vk_ic_kida_pelz": (_I, [_I, _D, _P]),
     "vk_ic_anti_parallel_tubes": (_I, [_I, _D, _D, _D, _D, _D, _P]),
     "vk_ic_random_isotropic_spectrum": (_I, [_I, _D, _D, _D, _U64, _P]),
+    "vk_ic_von_karman_spectrum": (_I, [_I, _D, _D, _D, _D, _U64, _P]),
     "vk_solver_nonlinear_vjp": (_I, [_P, _P, _P, _P]),
     "vk_solver_step_vjp": (_I, [_P, _P, _P]),
     "vk_solver_step_vjp_device": (_I, [_P, _P, _P]),

@@ -274,9 +274,11 @@ int launch_etd_table(double *tab, int qmax, double nu, double ks, const Scalars 
 // ---------------------------------------------------------------------------
 // Diagnostics (vonkarman-periodic/src/solver.rs:412-535, vonkarman-diag/src/
 // spectrum.rs:13-60): one pass over u computes the R2C-weighted sums of
-// |u|^2, |w|^2, u.w, w.curl(w) and the shell histogram of |u|^2.  Each thread
-// keeps Neumaier-compensated sums; blocks reduce in a fixed tree order and a
-// final block sums the per-block partials in fixed order (compensated).
+// |u|^2, |w|^2, u.w, w.curl(w) and, when the spectrum is asked for, the shell
+// histogram of |u|^2.  Each thread keeps Neumaier-compensated sums; blocks
+// reduce in a fixed tree order and a final block sums the per-block partials
+// in fixed order (compensated).  Every sum is therefore the same from run to
+// run: the histogram has no floating-point atomics either (below).
 // ---------------------------------------------------------------------------
 struct Neu {
     double s = 0.0, c = 0.0;
@@ -289,6 +291,7 @@ struct Neu {
 };
 
 constexpr int DIAG_NT = 256;
+constexpr int DIAG_WAVES = DIAG_NT / 64;
 constexpr int DIAG_MAXSH = 1024;
 
 __device__ __forceinline__ void curl_mode(double kx, double ky, double kz, const cpx<double> u[3], cpx<double> w[3]) {
@@ -298,41 +301,77 @@ __device__ __forceinline__ void curl_mode(double kx, double ky, double kz, const
     w[2] = {-(kx * u[1].im - ky * u[0].im), kx * u[1].re - ky * u[0].re};
 }
 
-template <typename TS>
+// Shell histogram step of one wave: each lane holds (shell, v) with shell -1
+// for none.  The wave visits its distinct shells in order of the first lane
+// holding each; for a shell, the lanes' values (zeros elsewhere) are summed by
+// a fixed xor-butterfly and lane 0 adds the sum to the wave's own LDS row.
+// The order of every floating-point addition is fixed by the data alone.
+__device__ __forceinline__ void shell_add(double *wrow, int shell, double v, int lane) {
+    unsigned long long rem = __ballot(shell >= 0);
+    while (rem) {
+        const int first = __ffsll((long long)rem) - 1;
+        const int s = __shfl(shell, first);
+        const bool mine = shell == s;
+        double x = mine ? v : 0.0;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+        if (lane == 0) wrow[s] += x;
+        rem &= ~__ballot(mine);
+    }
+}
+
+template <typename TS, bool SPEC>
 __global__ __launch_bounds__(DIAG_NT) void k_diag(const cpx<TS> *__restrict__ u, int n, int hp, long long fs,
                                                  double ks, int nshells, double *__restrict__ partials) {
-    __shared__ double spec[DIAG_MAXSH];
+    __shared__ double spec[SPEC ? DIAG_WAVES : 1][SPEC ? DIAG_MAXSH : 1];
     __shared__ double red[4][DIAG_NT];
-    const int tid = threadIdx.x;
-    for (int s = tid; s < nshells; s += DIAG_NT) spec[s] = 0.0;
-    __syncthreads();
+    const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+    if constexpr (SPEC) {
+        for (int s = tid; s < DIAG_WAVES * DIAG_MAXSH; s += DIAG_NT) spec[s / DIAG_MAXSH][s % DIAG_MAXSH] = 0.0;
+        __syncthreads();
+    }
     const int h = n / 2 + 1;
     const long long total = (long long)n * n * hp;
     Neu acc[4];
-    for (long long i = blockIdx.x * (long long)DIAG_NT + tid; i < total; i += (long long)gridDim.x * DIAG_NT) {
+    // the trip count is the same for every lane of a block (shell_add is a
+    // wave-wide operation); lanes past the end or in row padding add nothing
+    for (long long base = blockIdx.x * (long long)DIAG_NT; base < total; base += (long long)gridDim.x * DIAG_NT) {
+        const long long i = base + tid;
         const int iz = (int)(i % hp);
-        if (iz >= h) continue;
-        const long long r = i / hp;
-        const int iy = (int)(r % n), ix = (int)(r / n);
-        const double kx = double(wavenum(ix, n)) * ks, ky = double(wavenum(iy, n)) * ks, kz = double(iz) * ks;
-        const cpx<double> uu[3] = {cvt<double>(u[i]), cvt<double>(u[fs + i]), cvt<double>(u[2 * fs + i])};
-        cpx<double> w[3], cw[3];
-        curl_mode(kx, ky, kz, uu, w);
-        curl_mode(kx, ky, kz, w, cw);
-        const double wt = (iz == 0 || iz == n / 2) ? 1.0 : 2.0;
-        double e = 0.0, o = 0.0, hh = 0.0, ss = 0.0;
-        for (int c = 0; c < 3; ++c) {
-            e += uu[c].re * uu[c].re + uu[c].im * uu[c].im;
-            o += w[c].re * w[c].re + w[c].im * w[c].im;
-            hh += uu[c].re * w[c].re + uu[c].im * w[c].im;
-            ss += w[c].re * cw[c].re + w[c].im * cw[c].im;
+        const bool valid = i < total && iz < h;
+        int shell = -1;
+        double ve = 0.0;
+        if (valid) {
+            const long long r = i / hp;
+            const int iy = (int)(r % n), ix = (int)(r / n);
+            const double kx = double(wavenum(ix, n)) * ks, ky = double(wavenum(iy, n)) * ks, kz = double(iz) * ks;
+            const cpx<double> uu[3] = {cvt<double>(u[i]), cvt<double>(u[fs + i]), cvt<double>(u[2 * fs + i])};
+            cpx<double> w[3], cw[3];
+            curl_mode(kx, ky, kz, uu, w);
+            curl_mode(kx, ky, kz, w, cw);
+            const double wt = (iz == 0 || iz == n / 2) ? 1.0 : 2.0;
+            double e = 0.0, o = 0.0, hh = 0.0, ss = 0.0;
+            for (int c = 0; c < 3; ++c) {
+                e += uu[c].re * uu[c].re + uu[c].im * uu[c].im;
+                o += w[c].re * w[c].re + w[c].im * w[c].im;
+                hh += uu[c].re * w[c].re + uu[c].im * w[c].im;
+                ss += w[c].re * cw[c].re + w[c].im * cw[c].im;
+            }
+            acc[0].add(wt * e);
+            acc[1].add(wt * o);
+            acc[2].add(wt * hh);
+            acc[3].add(wt * ss);
+            if constexpr (SPEC) {
+                // f64::round (spectrum.rs:42): halves away from zero, which
+                // matters when 2 pi / l makes |k| a half-integer
+                const int sh = (int)round(sqrt(kx * kx + ky * ky + kz * kz));
+                if (sh < nshells) {
+                    shell = sh;
+                    ve = 0.5 * wt * e;
+                }
+            }
         }
-        acc[0].add(wt * e);
-        acc[1].add(wt * o);
-        acc[2].add(wt * hh);
-        acc[3].add(wt * ss);
-        const int shell = (int)rint(sqrt(kx * kx + ky * ky + kz * kz));
-        if (shell < nshells) atomicAdd(&spec[shell], 0.5 * wt * e);
+        if constexpr (SPEC) shell_add(spec[wave], shell, ve, lane);
     }
     for (int k = 0; k < 4; ++k) red[k][tid] = acc[k].val();
     __syncthreads();
@@ -343,32 +382,44 @@ __global__ __launch_bounds__(DIAG_NT) void k_diag(const cpx<TS> *__restrict__ u,
     }
     double *row = partials + (long long)blockIdx.x * (4 + nshells);
     if (tid < 4) row[tid] = red[tid][0];
-    for (int s = tid; s < nshells; s += DIAG_NT) row[4 + s] = spec[s];
+    if constexpr (SPEC) {
+        // the waves' rows merged in wave order
+        for (int s = tid; s < nshells; s += DIAG_NT) {
+            double t = spec[0][s];
+            for (int w = 1; w < DIAG_WAVES; ++w) t += spec[w][s];
+            row[4 + s] = t;
+        }
+    }
 }
 
 __global__ void k_diag_final(const double *__restrict__ partials, int nblocks, int nshells, double inv_nt2,
                              double *__restrict__ spec_out, Scalars *sc) {
-    const int ncol = 4 + nshells;
+    const int ncol = spec_out ? 4 + nshells : 4;
+    const int stride = 4 + nshells;
     for (int c = threadIdx.x; c < ncol; c += blockDim.x) {
         Neu a;
         // block order is kept; the unrolled loads are in flight together
 #pragma unroll 16
-        for (int b = 0; b < nblocks; ++b) a.add(partials[(long long)b * ncol + c]);
+        for (int b = 0; b < nblocks; ++b) a.add(partials[(long long)b * stride + c]);
         const double v = a.val() * inv_nt2;
         if (c < 4) sc->diag[c] = c == 0 ? 0.5 * v : v;
-        else if (spec_out) spec_out[c - 4] = v;
+        else spec_out[c - 4] = v;
     }
 }
 
 int launch_diag(const void *u, Prec sp, int n, int hp, long long fs, double ks, int nshells, double *partials,
                 int nblocks, double *spec_out, Scalars *sc, hipStream_t st) {
     if (nshells > DIAG_MAXSH) return 2;
-    if (sp == F64)
-        hipLaunchKernelGGL((k_diag<double>), dim3(nblocks), dim3(DIAG_NT), 0, st,
-                           reinterpret_cast<const cpx<double> *>(u), n, hp, fs, ks, nshells, partials);
-    else
-        hipLaunchKernelGGL((k_diag<float>), dim3(nblocks), dim3(DIAG_NT), 0, st,
-                           reinterpret_cast<const cpx<float> *>(u), n, hp, fs, ks, nshells, partials);
+    const dim3 g(nblocks), b(DIAG_NT);
+    if (sp == F64) {
+        const auto *p = reinterpret_cast<const cpx<double> *>(u);
+        if (spec_out) hipLaunchKernelGGL((k_diag<double, true>), g, b, 0, st, p, n, hp, fs, ks, nshells, partials);
+        else hipLaunchKernelGGL((k_diag<double, false>), g, b, 0, st, p, n, hp, fs, ks, nshells, partials);
+    } else {
+        const auto *p = reinterpret_cast<const cpx<float> *>(u);
+        if (spec_out) hipLaunchKernelGGL((k_diag<float, true>), g, b, 0, st, p, n, hp, fs, ks, nshells, partials);
+        else hipLaunchKernelGGL((k_diag<float, false>), g, b, 0, st, p, n, hp, fs, ks, nshells, partials);
+    }
     const double nt = (double)n * n * n;
     hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(256), 0, st, partials, nblocks, nshells, 1.0 / (nt * nt), spec_out,
                        sc);

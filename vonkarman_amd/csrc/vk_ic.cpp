@@ -218,4 +218,103 @@ int vk_ic_random_isotropic_spectrum(int n, double l, double k_peak, double total
     return VK_OK;
 }
 
+// The von Karman isotropic field of the benchmark configurations (no
+// reference counterpart).  Per mode k != 0 and component, amplitude
+// sqrt(E(|k|) / (4 pi |k|^2)) with
+//   E(k) = (k/k0)^4 / (1 + (k/k0)^2)^(17/6) * exp(-2 (k/kd)^2),
+// so the shell sums follow E(k), and phase 2 pi r with r from xorshift64 in
+// random_isotropic.rs's generator and draw order (modes in storage order,
+// three draws per mode, none for k = 0).  Then the Leray projection,
+// Hermitian symmetrisation of the kz = 0 and kz = n/2 planes (each pair
+// (kx, ky), (-kx, -ky) of a plane set to the mean of the one and the other's
+// conjugate, self-paired modes to their real part), and scaling to
+// total_energy under the R2C-weighted norm (compensated sum).
+int vk_ic_von_karman_spectrum(int n, double l, double k0, double kd, double total_energy, uint64_t seed,
+                              double *u_hat) {
+    if (n < 2 || (n % 2) || !(l > 0.0) || !(k0 > 0.0) || !(kd > 0.0) || !u_hat || seed == 0)
+        return vk::set_error(VK_ERR_ARG, "vk_ic_von_karman_spectrum: bad argument (n even >= 2, l, k0, kd > 0, seed != 0)");
+    const int h = n / 2 + 1;
+    const size_t S = (size_t)n * n * h;
+    const double scale = 2.0 * M_PI / l;
+    std::vector<double> kf(n), kh(h);
+    for (int i = 0; i < n; ++i) kf[i] = (double)(i < n / 2 ? i : i - n) * scale;
+    for (int i = 0; i < h; ++i) kh[i] = (double)i * scale;
+    uint64_t state = seed;
+    auto next = [&]() {
+        state ^= state << 13;
+        state ^= state >> 7;
+        state ^= state << 17;
+        return (double)state / 18446744073709551615.0;
+    };
+    auto re = [&](int c, size_t m) -> double & { return u_hat[2 * (c * S + m)]; };
+    auto im = [&](int c, size_t m) -> double & { return u_hat[2 * (c * S + m) + 1]; };
+    for (int ix = 0; ix < n; ++ix)
+        for (int iy = 0; iy < n; ++iy)
+            for (int iz = 0; iz < h; ++iz) {
+                const size_t m = ((size_t)ix * n + iy) * h + iz;
+                const double k2 = kf[ix] * kf[ix] + kf[iy] * kf[iy] + kh[iz] * kh[iz];
+                if (k2 < 1e-30) {
+                    for (int c = 0; c < 3; ++c) re(c, m) = im(c, m) = 0.0;
+                    continue;
+                }
+                const double k = std::sqrt(k2), r = k / k0, q = k / kd;
+                const double ek = (r * r) * (r * r) / std::pow(1.0 + r * r, 17.0 / 6.0) * std::exp(-2.0 * q * q);
+                const double amp = std::sqrt(ek / (4.0 * M_PI * k2));
+                for (int c = 0; c < 3; ++c) {
+                    const double phase = 2.0 * M_PI * next();
+                    re(c, m) = amp * std::cos(phase);
+                    im(c, m) = amp * std::sin(phase);
+                }
+            }
+    // Leray projection (spectral_ops.rs:114-151)
+    for (int ix = 0; ix < n; ++ix)
+        for (int iy = 0; iy < n; ++iy)
+            for (int iz = 0; iz < h; ++iz) {
+                const double kx = kf[ix], ky = kf[iy], kz = kh[iz];
+                const double k2 = kx * kx + ky * ky + kz * kz;
+                if (k2 < 1e-30) continue;
+                const size_t m = ((size_t)ix * n + iy) * h + iz;
+                const double dr = (kx * re(0, m) + ky * re(1, m) + kz * re(2, m)) / k2;
+                const double di = (kx * im(0, m) + ky * im(1, m) + kz * im(2, m)) / k2;
+                const double kk[3] = {kx, ky, kz};
+                for (int c = 0; c < 3; ++c) {
+                    re(c, m) -= kk[c] * dr;
+                    im(c, m) -= kk[c] * di;
+                }
+            }
+    // Hermitian symmetrisation of the kz = 0 and kz = n/2 planes
+    for (int iz : {0, n / 2})
+        for (int ix = 0; ix < n; ++ix)
+            for (int iy = 0; iy < n; ++iy) {
+                const int jx = (n - ix) % n, jy = (n - iy) % n;
+                if (jx * n + jy < ix * n + iy) continue;  // each pair once
+                const size_t a = ((size_t)ix * n + iy) * h + iz, b = ((size_t)jx * n + jy) * h + iz;
+                for (int c = 0; c < 3; ++c) {
+                    const double sr = 0.5 * (re(c, a) + re(c, b)), si = 0.5 * (im(c, a) - im(c, b));
+                    re(c, a) = sr;
+                    im(c, a) = si;
+                    re(c, b) = sr;
+                    im(c, b) = -si;
+                }
+            }
+    // R2C-weighted energy, Neumaier-compensated
+    double e = 0.0, comp = 0.0;
+    for (int c = 0; c < 3; ++c)
+        for (size_t m = 0; m < S; ++m) {
+            const int iz = (int)(m % h);
+            const double w = (iz == 0 || iz == n / 2) ? 1.0 : 2.0;
+            const double x = w * (re(c, m) * re(c, m) + im(c, m) * im(c, m));
+            const double t = e + x;
+            comp += std::fabs(e) >= std::fabs(x) ? (e - t) + x : (x - t) + e;
+            e = t;
+        }
+    const double ntot = (double)n * (double)n * (double)n;
+    e = 0.5 * (e + comp) / (ntot * ntot);
+    if (e > 1e-300) {
+        const double s = std::sqrt(total_energy / e);
+        for (size_t i = 0; i < 2 * 3 * S; ++i) u_hat[i] *= s;
+    }
+    return VK_OK;
+}
+
 }  // extern "C"

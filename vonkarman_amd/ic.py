@@ -10,7 +10,7 @@ projection and energy scaling); ``random_isotropic`` applies that C2R on the
 device.
 
 ``von_karman_isotropic`` and ``lamb_oseen_tubes`` are the initial fields of
-this backend's benchmark configurations (a von Karman spectrum with PCG64
+this backend's benchmark configurations (a von Karman spectrum with xorshift64
 phases, and tubes defined by their vorticity); the reference ships neither.
 """
 from __future__ import annotations
@@ -93,65 +93,33 @@ def _leray(uh: np.ndarray, kx, ky, kz) -> np.ndarray:
     return np.stack([uh[0] - KX * kdu * inv, uh[1] - KY * kdu * inv, uh[2] - KZ * kdu * inv])
 
 
-def _energy(uh: np.ndarray, n: int) -> float:
-    w = np.full(n // 2 + 1, 2.0)
-    w[0] = 1.0
-    w[n // 2] = 1.0
-    nt = float(n) ** 3
-    return 0.5 * float(np.sum(w * (uh.real ** 2 + uh.imag ** 2))) / (nt * nt)
-
-
-def _hermitian_plane(p: np.ndarray) -> np.ndarray:
-    n = p.shape[0]
-    neg = (-np.arange(n)) % n
-    return 0.5 * (p + np.conj(p[np.ix_(neg, neg)]))
-
-
 def von_karman_isotropic(n: int, k0: float = 4.0, kd: float = 64.0, energy: float = 0.5,
-                         seed: int = 42) -> np.ndarray:
+                         seed: int = 42, l: float = 2.0 * math.pi) -> np.ndarray:
     """Random isotropic spectral velocity with the von Karman spectrum
     E(k) = (k/k0)^4 / (1+(k/k0)^2)^(17/6) * exp(-2 (k/kd)^2).
 
-    Per mode k != 0 and component, amplitude sqrt(E(|k|) / (4 pi |k|^2)) -- so
-    the shell-summed spectrum follows E(k) -- with a uniform random phase
-    (numpy PCG64 seeded with `seed`, drawn in C order of (c, ix, iy, iz));
-    then Leray projection, Hermitian symmetrisation of the kz=0 plane, the
-    Nyquist modes (any index n/2) set to zero, and scaling to total energy
-    `energy` under the R2C-weighted norm of solver.rs:412-434.
-    Returns complex128[3, n, n, n//2+1].
+    Per mode k != 0 and component, amplitude sqrt(E(|k|) / (4 pi |k|^2)) --
+    so the shell-summed spectrum follows E(k) -- and phase 2 pi r, r drawn by
+    xorshift64 from `seed` in the generator and draw order of
+    random_isotropic.rs:30-64 (three draws per mode, modes in storage order);
+    then the Leray projection, Hermitian symmetrisation of the kz = 0 and
+    kz = n/2 planes, and scaling to total energy `energy` under the
+    R2C-weighted norm of solver.rs:412-434.  Host code of the library
+    (vk_ic_von_karman_spectrum).  Returns complex128[3, n, n, n//2+1].
     """
-    kx, ky, kz = _wavenumbers(n)
-    KX, KY, KZ = np.meshgrid(kx, ky, kz, indexing="ij")
-    k = np.sqrt(KX * KX + KY * KY + KZ * KZ)
-    r = k / k0
-    spec = r ** 4 / (1.0 + r * r) ** (17.0 / 6.0) * np.exp(-2.0 * (k / kd) ** 2)
-    with np.errstate(divide="ignore", invalid="ignore"):
-        amp = np.sqrt(spec / (4.0 * np.pi * k * k))
-    amp[0, 0, 0] = 0.0
-    rng = np.random.Generator(np.random.PCG64(seed))
-    phase = 2.0 * np.pi * rng.random((3,) + k.shape)
-    uh = amp[None] * np.exp(1j * phase)
-    uh = _leray(uh, kx, ky, kz)
-    for c in range(3):
-        uh[c, :, :, 0] = _hermitian_plane(uh[c, :, :, 0])
-    h = n // 2
-    uh[:, h, :, :] = 0.0
-    uh[:, :, h, :] = 0.0
-    uh[:, :, :, h] = 0.0
-    e = _energy(uh, n)
-    if e > 1e-30:
-        uh *= math.sqrt(energy / e)
-    return np.ascontiguousarray(uh)
+    out = np.empty((3, n, n, n // 2 + 1), np.complex128)
+    check(lib().vk_ic_von_karman_spectrum(n, l, k0, kd, energy, int(seed) & (2**64 - 1), ptr(out)),
+          "vk_ic_von_karman_spectrum")
+    return out
 
 
-def lamb_oseen_tubes(n: int, core_radius: float = 0.3, circulation: float = 1.0,
-                     offset: float = 0.8, amplitude: float = 0.1) -> np.ndarray:
-    """Two anti-parallel Lamb-Oseen vortex tubes along x (Gaussian vorticity
-    profile, circulation +-Gamma, core radius a) centred at y = pi +- offset,
-    z = pi, with the centreline displaced by amplitude*cos(x) in y.  The
-    vorticity omega_x(x, y, z) = sum_tubes +-Gamma/(pi a^2) exp(-r^2/a^2) (with
-    periodic images in y and z) is transformed, Leray-projected and converted
-    to velocity u_hat = i k x omega_hat / |k|^2.  Returns complex128[3, n, n, n//2+1]."""
+def lamb_oseen_vorticity(n: int, core_radius: float = 0.3, circulation: float = 1.0, offset: float = 0.8,
+                         amplitude: float = 0.1) -> np.ndarray:
+    """omega_x on the n^3 grid of [0, 2 pi)^3 for two anti-parallel
+    Lamb-Oseen tubes along x: sum over tubes s = +1 (y = pi + offset) and
+    s = -1 (y = pi - offset), z = pi, centreline y shifted by
+    amplitude*cos(x), and their periodic images in y and z, of
+    s Gamma / (pi a^2) exp(-r^2 / a^2).  Returns float64[n, n, n]."""
     L = 2.0 * math.pi
     x = np.arange(n) * L / n
     X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
@@ -163,16 +131,23 @@ def lamb_oseen_tubes(n: int, core_radius: float = 0.3, circulation: float = 1.0,
                 dy = Y - yc_x + iy * L
                 dz = Z - math.pi + iz * L
                 wx += sign * circulation / (math.pi * core_radius ** 2) * np.exp(-(dy * dy + dz * dz) / core_radius ** 2)
-    from .fft import HipFftBackend
+    return wx
 
-    wh = np.zeros((3, n, n, n // 2 + 1), np.complex128)
-    HipFftBackend(n, n, n).r2c_3d(np.ascontiguousarray(wx), wh[0])
+
+def velocity_from_vorticity(wx_hat: np.ndarray) -> np.ndarray:
+    """u_hat from the spectrum of a vorticity field (omega_x, 0, 0): Leray
+    projection of omega, then u_hat = i k x omega_hat / |k|^2 (the inverse of
+    the curl on divergence-free fields, spectral_ops.rs:81-108), with every
+    mode of a Nyquist index set to zero.  Host arithmetic on the spectrum.
+    Returns complex128[3, n, n, n//2+1]."""
+    n = wx_hat.shape[0]
+    wh = np.zeros((3,) + wx_hat.shape, np.complex128)
+    wh[0] = wx_hat
     kx, ky, kz = _wavenumbers(n)
     wh = _leray(wh, kx, ky, kz)
     KX, KY, KZ = np.meshgrid(kx, ky, kz, indexing="ij")
     k2 = KX * KX + KY * KY + KZ * KZ
     inv = np.where(k2 < 1e-30, 0.0, 1.0 / np.where(k2 < 1e-30, 1.0, k2))
-    # u_hat = i k x w_hat / k^2  (curl(u) = w for divergence-free u)
     uh = np.stack([
         1j * (KY * wh[2] - KZ * wh[1]) * inv,
         1j * (KZ * wh[0] - KX * wh[2]) * inv,
@@ -183,3 +158,18 @@ def lamb_oseen_tubes(n: int, core_radius: float = 0.3, circulation: float = 1.0,
     uh[:, :, h, :] = 0.0
     uh[:, :, :, h] = 0.0
     return np.ascontiguousarray(uh)
+
+
+def lamb_oseen_tubes(n: int, core_radius: float = 0.3, circulation: float = 1.0,
+                     offset: float = 0.8, amplitude: float = 0.1) -> np.ndarray:
+    """Two anti-parallel Lamb-Oseen vortex tubes along x (Gaussian vorticity
+    profile, circulation +-Gamma, core radius a) centred at y = pi +- offset,
+    z = pi, with the centreline displaced by amplitude*cos(x) in y:
+    ``lamb_oseen_vorticity`` transformed by the device R2C, then
+    ``velocity_from_vorticity``.  Returns complex128[3, n, n, n//2+1]."""
+    from .fft import HipFftBackend
+
+    wx = lamb_oseen_vorticity(n, core_radius, circulation, offset, amplitude)
+    wxh = np.empty((n, n, n // 2 + 1), np.complex128)
+    HipFftBackend(n, n, n).r2c_3d(np.ascontiguousarray(wx), wxh)
+    return velocity_from_vorticity(wxh)

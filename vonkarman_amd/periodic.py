@@ -169,7 +169,7 @@ class Periodic3D:
                 fft.r2c_3d(u[c], uh[c])
             return uh
         if isinstance(ic, IcType.VonKarmanIsotropic):
-            return _ic.von_karman_isotropic(n, ic.k0, ic.kd, ic.energy, ic.seed)
+            return _ic.von_karman_isotropic(n, ic.k0, ic.kd, ic.energy, ic.seed, l)
         if isinstance(ic, IcType.LambOseenTubes):
             return _ic.lamb_oseen_tubes(n, ic.core_radius, ic.circulation, ic.offset, ic.amplitude)
         if isinstance(ic, IcType.Spectral):

@@ -17,8 +17,14 @@ container of ``checkpoint.py`` (named ``checkpoint_{step:08}.npz`` and
 ``IcType`` tags of vonkarman-periodic/src/ic/mod.rs:16-18 with their default
 parameters (the reference's loop accepts only ``taylor-green``); and
 ``precision.tier = "mixed"`` selects the mixed mode, where the reference reads
-no precision setting.  There is no command line here: the caller passes a
-parsed config.
+no precision setting.
+
+``validate_config`` is vonkarman-bin/src/validate.rs:6-57 (with "hip" among
+the backend names and the precision tier checked), and ``main`` is the
+``vonkarman run --config FILE [--backend NAME] [--restart CKPT]`` command of
+vonkarman-bin/src/main.rs:19-66:
+
+    python -m vonkarman_amd.run run --config experiment.toml --backend hip
 """
 from __future__ import annotations
 
@@ -191,6 +197,41 @@ class ExperimentConfig:
 
 
 # ---------------------------------------------------------------------------
+# validate_config (vonkarman-bin/src/validate.rs:6-57)
+# ---------------------------------------------------------------------------
+# the reference's backend names (validate.rs:37) and this backend's
+VALID_BACKENDS = ("auto", "cufft", "gpu", "cuda", "cpu", "ndrustfft", "hip")
+PRECISION_TIERS = ("f64", "mixed")
+
+
+def validate_config(config: "ExperimentConfig") -> list[str]:
+    """Every error of the config, in the reference's order (empty = valid):
+    n a power of two >= 8, nu > 0, l > 0, at least one termination
+    condition, a known backend name (case-insensitive), cfl_safety in
+    (0, 1] when an [integrator] table is given, and a known precision tier."""
+    errors = []
+    n = config.domain.n
+    if n < 8 or n & (n - 1):
+        errors.append(f"domain.n must be a power of 2 and >= 8, got {n}")
+    if not config.physics.nu > 0.0:
+        errors.append(f"physics.nu must be > 0, got {config.physics.nu}")
+    if not config.domain.l > 0.0:
+        errors.append(f"domain.l must be > 0, got {config.domain.l}")
+    te = config.termination
+    if te.max_steps is None and te.max_time is None and te.max_wall_hours is None:
+        errors.append("at least one termination condition required (max_steps, max_time, or max_wall_hours)")
+    if config.domain.backend.lower() not in VALID_BACKENDS:
+        errors.append(f"domain.backend must be one of {list(VALID_BACKENDS)}, got \"{config.domain.backend}\"")
+    it = config.integrator
+    if it is not None and not (0.0 < it.cfl_safety <= 1.0):
+        errors.append(f"integrator.cfl_safety must be in (0, 1], got {it.cfl_safety}")
+    pr = config.precision
+    if pr is not None and pr.tier not in PRECISION_TIERS:
+        errors.append(f"precision.tier must be one of {list(PRECISION_TIERS)}, got \"{pr.tier}\"")
+    return errors
+
+
+# ---------------------------------------------------------------------------
 # ConservationAudit (vonkarman-diag/src/audit.rs:1-205)
 # ---------------------------------------------------------------------------
 @dataclass
@@ -300,6 +341,9 @@ def run(config: ExperimentConfig, restart: Optional[str] = None,
     ``solver_factory(grid, nu, ic, backend_mode, precision)`` builds the
     Domain (default: the hip ``Periodic3D``); a restart reads the checkpoint
     and continues its clock, dt and step count (solver.rs:111-159)."""
+    errors = validate_config(config)
+    if errors:
+        raise ConfigError("configuration errors:\n" + "\n".join(f"  - {e}" for e in errors))
     grid = GridSpec.cubic(config.domain.n, config.domain.l)
     nu = config.physics.nu
     log.info("starting simulation name=%s n=%d nu=%g re=%g", config.run.name, config.domain.n, nu,
@@ -430,4 +474,37 @@ def run(config: ExperimentConfig, restart: Optional[str] = None,
                       list(audit.violations))
 
 
-__all__ = ["ExperimentConfig", "ConservationAudit", "AuditConfig", "Violation", "RunSummary", "run", "ConfigError"]
+def main(argv: Optional[list] = None) -> int:
+    """vonkarman-bin/src/main.rs:19-66: ``run --config FILE [--backend NAME]
+    [--restart CKPT]``.  --backend overrides domain.backend; the config is
+    validated before the solver is built, and its errors are printed one per
+    line with exit status 1."""
+    import argparse
+    import sys
+
+    ap = argparse.ArgumentParser(prog="vonkarman", description="Multi-precision pseudospectral Navier-Stokes solver")
+    sub = ap.add_subparsers(dest="command", required=True)
+    pr = sub.add_parser("run", help="Run a simulation from a TOML config file.")
+    pr.add_argument("-c", "--config", required=True, help="Path to the experiment TOML config.")
+    pr.add_argument("--backend", default="", help="FFT backend (auto, hip, ...). Overrides TOML config.")
+    pr.add_argument("--restart", default=None, help="Path to a checkpoint file to restart from.")
+    args = ap.parse_args(argv)
+    logging.basicConfig(level=logging.INFO, format="%(asctime)s %(levelname)s %(name)s: %(message)s")
+    config = ExperimentConfig.from_file(args.config)
+    if args.backend:
+        config.domain.backend = args.backend
+    errors = validate_config(config)
+    if errors:
+        print("Configuration errors:", file=sys.stderr)
+        for e in errors:
+            print(f"  - {e}", file=sys.stderr)
+        return 1
+    run(config, args.restart)
+    return 0
+
+
+__all__ = ["ExperimentConfig", "ConservationAudit", "AuditConfig", "Violation", "RunSummary", "run", "ConfigError",
+           "validate_config", "main"]
+
+if __name__ == "__main__":
+    raise SystemExit(main())
