@@ -65,6 +65,19 @@ int vk_fft_c2r_3d(vk_fft_plan *plan, const double *in, double *out);
 /* The same two transforms on device pointers (same layouts). */
 int vk_fft_r2c_3d_device(vk_fft_plan *plan, const double *d_in, double *d_out);
 int vk_fft_c2r_3d_device(vk_fft_plan *plan, const double *d_in, double *d_out);
+/* No counterpart in the reference: vk_fft_c2r_3d_device without a plan and
+ * without allocating device memory, for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same layouts, semantics and results as vk_fft_c2r_3d_device.
+ * The intermediates live in one fixed 64 MiB device array per GPU, so
+ * nx*ny*hp*16 B must not exceed 32 MiB (hp = nz/2+1 rounded up to a multiple
+ * of 8; 128^3 fits); larger grids return VK_ERR_ARG.  The kernels are
+ * enqueued on `stream` (a hipStream_t; NULL for the null stream) and the call
+ * returns without waiting for them.  Calls that overlap on one GPU share its
+ * array and race.  The first call on each device also looks up the array's
+ * device address and uploads the twiddle tables, synchronously: make it
+ * outside stream capture. */
+int vk_fft_c2r_3d_device_noalloc(int nx, int ny, int nz, const double *d_in, double *d_out, void *stream);
 /* FftBackend::name / precision_digits (backend.rs:21-24). */
 const char *vk_fft_name(const vk_fft_plan *plan);
 int vk_fft_precision_digits(const vk_fft_plan *plan);

@@ -84,6 +84,7 @@ SIGNATURES = {
     "vk_fft_c2r_3d": (_I, [_P, _P, _P]),
     "vk_fft_r2c_3d_device": (_I, [_P, _P, _P]),
     "vk_fft_c2r_3d_device": (_I, [_P, _P, _P]),
+    "vk_fft_c2r_3d_device_noalloc": (_I, [_I, _I, _I, _P, _P, _P]),
     "vk_fft_name": (ctypes.c_char_p, [_P]),
     "vk_fft_precision_digits": (_I, [_P]),
     "vk_fft_last_time_ms": (_I, [_P, _PD]),

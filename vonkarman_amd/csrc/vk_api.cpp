@@ -147,21 +147,37 @@ struct vk_fft_plan {
     std::string name;
 };
 
-static int plan_c2r_kernels(vk_fft_plan *p, const void *spec) {
-    // x inverse: the caller's spectrum [ix][iy][hz] (rows of hz, unpadded) ->
-    // t2 [iy][ix][hp].  Column (iy, iz) of the padded column space reads
-    // iy*hz + iz; padding columns (iz >= hz) read nothing.
-    const int nx = p->nx, ny = p->ny, nz = p->nz, hp = p->hp, hz = p->hz;
+namespace vk {
+// The unnormalised-spectrum C2R of an nx*ny*nz grid (FftBackend::c2r_3d,
+// backend.rs:15-18), enqueued on `st`: spec is the caller's complex
+// [ix][iy][hz] array (rows of hz = nz/2+1, unpadded), t1 and t2 are
+// intermediates of nx*ny*hp complex (hp = hz rounded up to whole 128-B rows),
+// real the output [ix][iy][nz].  Used by the plan's entry points and the
+// allocation-free one.
+int fft_c2r_launch(int nx, int ny, int nz, int hp, const void *spec, void *t1, void *t2, double *real,
+                   const void *tw_x, const void *tw_y, const void *tw_z, hipStream_t st) {
+    const int hz = nz / 2 + 1;
+    // x inverse: the spectrum [ix][iy][hz] -> t2 [iy][ix][hp].  Column (iy, iz)
+    // of the padded column space reads iy*hz + iz; padding columns (iz >= hz)
+    // read nothing.
     Strided gx = make_strided(ny * hp, hp, hz, hp, hz, (long long)ny * hz, 0, hp, (long long)nx * hp, hp, 0);
-    VK_TRY(launch_inv(INV_PLAIN, nx, nx, F64, F64, spec, p->t2.p, p->tw_x.p, gx, nx, 1.0, 1.0, p->st));
+    VK_TRY(launch_inv(INV_PLAIN, nx, nx, F64, F64, spec, t2, tw_x, gx, nx, 1.0, 1.0, st));
     // y inverse: t2 [iy][ix][hp] -> t1 [ix][iy][hp]
     Strided gy = make_strided(nx * hp, hp, hz, nx * hp, 0, (long long)nx * hp, 0, hp, (long long)ny * hp, hp, 0);
-    VK_TRY(launch_inv(INV_PLAIN, ny, ny, F64, F64, p->t2.p, p->t1.p, p->tw_y.p, gy, ny, 1.0, 1.0, p->st));
+    VK_TRY(launch_inv(INV_PLAIN, ny, ny, F64, F64, t2, t1, tw_y, gy, ny, 1.0, 1.0, st));
     // z C2R with the 1/(nx ny nz) normalisation
     const double scale = 1.0 / ((double)nx * (double)ny * (double)nz);
-    VK_TRY(launch_z_c2r(nz, F64, p->t1.p, static_cast<double *>(p->real.p), (long long)nx * ny, hp, scale, p->tw_z.p,
-                        p->st));
+    VK_TRY(launch_z_c2r(nz, F64, t1, real, (long long)nx * ny, hp, scale, tw_z, st));
     return 0;
+}
+
+std::vector<double> host_twiddles(int L) { return twiddles(L); }
+bool fft_length_ok(int n) { return length_ok(n); }
+}  // namespace vk
+
+static int plan_c2r_kernels(vk_fft_plan *p, const void *spec) {
+    return fft_c2r_launch(p->nx, p->ny, p->nz, p->hp, spec, p->t1.p, p->t2.p, static_cast<double *>(p->real.p),
+                          p->tw_x.p, p->tw_y.p, p->tw_z.p, p->st);
 }
 
 static int plan_r2c_kernels(vk_fft_plan *p) {
