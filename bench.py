@@ -554,27 +554,158 @@ def measure_traffic(n: int, pass_name: str, timeout: float = 150.0) -> dict:
             "fetch_scale": fetch_scale, "write_scale": write_scale, "kernel": want}
 
 
+def _host_cpu() -> dict:
+    """The host's CPU model (/proc/cpuinfo) and usable core count."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
+    return {"model": model, "nproc": usable, "cpu_count": os.cpu_count()}
+
+
 def cpu_baseline(n: int) -> dict:
     """The oracle (C++ f64 restatement of the reference's CPU path) on the
-    host: one whole adaptive ETD-RK4 step of the benchmark workload at n^3
-    (CFL transforms, four nonlinear evaluations, stage combines), all host
-    threads up to 16."""
+    host, on the benchmark workload's initial field at n^3.
+
+    The reference solver is serial (rayon is declared but unused), so the
+    headline value is single-threaded.  A serial 256^3 step takes minutes, so
+    it is sampled: one compute_nonlinear (9 padded 3D transforms, u x w,
+    truncation, projection) and one CFL dt evaluation (3 unpadded C2Rs and
+    the max), timed on 1 thread; a step is 4 nonlinear evaluations and one CFL
+    evaluation, plus stage combines that the sample omits (a few percent), so
+    the value is an upper bound on the serial steps/s.  Beside it, one whole
+    adaptive step measured on all host threads up to 16."""
     from oracle import oracle as O
     from vonkarman_amd import ic as vic
 
-    threads = min(16, os.cpu_count() or 1)
-    O.lib().or_set_threads(threads)
+    host = _host_cpu()
     u0 = vic.von_karman_isotropic(n, seed=42)
+    O.lib().or_set_threads(1)
+    t0 = time.perf_counter()
+    O.nonlinear(u0)
+    t_nl = time.perf_counter() - t0
     r = O.Solver(n, 1e-3)
     r.set_state(u0)
+    t0 = time.perf_counter()
+    r.cfl_dt()
+    t_cfl = time.perf_counter() - t0
+    t_step1 = 4.0 * t_nl + t_cfl
+    threads = min(16, host["nproc"])
+    O.lib().or_set_threads(threads)
     r.init_dt()
     t0 = time.perf_counter()
     r.step(1)
-    t = time.perf_counter() - t0
+    t_multi = time.perf_counter() - t0
     del r
-    return {"value": 1.0 / t, "unit": "steps/s", "cores": threads, "kind": "port",
-            "sample": f"one adaptive ETD-RK4 step at {n}^3 FP64 of the same workload (oracle, {threads} "
-                      f"threads): {t:.2f} s"}
+    return {"value": 1.0 / t_step1, "unit": "steps/s", "cores": 1, "kind": "port",
+            "sample": f"1 thread, {n}^3 of the same workload: one compute_nonlinear {t_nl:.2f} s and one CFL dt "
+                      f"{t_cfl:.2f} s, step = 4 x nonlinear + CFL = {t_step1:.1f} s (stage combines omitted)",
+            "host_cpu": host,
+            "multi_thread": {"value": 1.0 / t_multi, "unit": "steps/s", "cores": threads,
+                             "sample": f"one whole adaptive ETD-RK4 step at {n}^3 measured on {threads} threads: "
+                                       f"{t_multi:.2f} s"}}
+
+
+def config_evidence(n: int = 256) -> dict:
+    """The accuracy and adjoint configurations, bounded for the default run:
+    mixed precision against FP64 over 1000 fixed-dt steps at 256^3; the 128^3
+    tubes gradient against central differences over the full 200-step
+    horizon; the 256^3 mixed trajectory gradient against FP64 over a
+    shortened horizon."""
+    import numpy as np
+
+    from vonkarman_amd import GridSpec, IcType, Periodic3D
+    from vonkarman_amd import ic as vic
+
+    res = {}
+    # mixed vs FP64, 1000 steps at dt = 5e-4, nu = 1e-3
+    res["mixed_1000_steps"] = workload_mixed_error(sizes=(n,), nsteps=1000)[str(n)]
+    # anti-parallel tubes at 128^3: 200 steps, dt = 2e-3, J = final
+    # enstrophy, a snapshot every 20 steps; fourth-order central differences
+    # (h = 1e-4 relative) along 8 seeded divergence-free directions, and the
+    # configuration's second-order difference at eps = 1e-6
+    nt, nu_t, dt_t, T = 128, 1e-3, 2e-3, 200
+    s = Periodic3D(GridSpec.cubic(nt), nu_t, IcType.LambOseenTubes(), init_dt=False)
+    s.set_fixed_dt(dt_t)
+    u0 = s.u_hat()
+    s.steps(2)
+    s.set_u_hat(u0)
+    s.set_clock(0.0, 0)
+    t0 = time.perf_counter()
+    s.steps(T)
+    s.energy()
+    t_fwd = time.perf_counter() - t0
+    s.set_u_hat(u0)
+    s.set_clock(0.0, 0)
+    t0 = time.perf_counter()
+    J, g, fsteps = s.adjoint(T, "final_enstrophy", max_snapshots=T // 20)
+    t_adj = time.perf_counter() - t0
+
+    def J_of(u):
+        s.set_u_hat(u)
+        s.set_clock(0.0, 0)
+        s.steps(T)
+        return s.enstrophy()
+
+    fd4, fd2 = [], []
+    for i in range(8):
+        d = _solenoidal_direction(nt, 1000 + i)
+        adv = float(np.vdot(g, d).real)
+        f4 = _central_difference(J_of, u0, d)
+        eps = 1e-6 * float(np.linalg.norm(u0)) / float(np.linalg.norm(d))
+        f2 = (J_of(u0 + eps * d) - J_of(u0 - eps * d)) / (2 * eps)
+        fd4.append(abs(f4 - adv) / abs(f4))
+        fd2.append(abs(f2 - adv) / abs(f2))
+    res["tubes_128_adjoint"] = {"steps": T, "dt": dt_t, "snapshots": T // 20, "J": J,
+                                "forward_s": t_fwd, "gradient_s": t_adj, "gradient_over_forward": t_adj / t_fwd,
+                                "forward_steps_in_gradient": fsteps,
+                                "fd4_rel_err": fd4, "fd4_max_rel_err": max(fd4),
+                                "fd2_eps1e-6_rel_err": fd2, "fd2_max_rel_err": max(fd2)}
+    print(f"[bench] tubes 128: fd4 max {max(fd4):.2e}, fd2 max {max(fd2):.2e}, gradient/forward "
+          f"{t_adj / t_fwd:.2f}", file=sys.stderr, flush=True)
+    del s
+    # mixed trajectory gradient at 256^3 over 100 steps (the configuration
+    # runs 2000): dt = 5e-4, J = sum_n dt 2 nu Omega(u_n), <= 40 snapshots
+    Tm, dt_m = 100, 5e-4
+    u0 = vic.von_karman_isotropic(n, k0=4.0, kd=64.0, energy=0.5, seed=42)
+    out = {}
+    for prec in ("mixed", "f64"):
+        s = Periodic3D(GridSpec.cubic(n), 1e-3, IcType.Spectral(u0), precision=prec, init_dt=False)
+        s.set_fixed_dt(dt_m)
+        s.steps(2)
+        s.set_u_hat(u0)
+        s.set_clock(0.0, 0)
+        t0 = time.perf_counter()
+        s.steps(Tm)
+        s.energy()
+        t_fwd = time.perf_counter() - t0
+        s.set_u_hat(u0)
+        s.set_clock(0.0, 0)
+        t0 = time.perf_counter()
+        Jm, gm, fs = s.adjoint(Tm, "dissipation", max_snapshots=40)
+        t_adj = time.perf_counter() - t0
+        out[prec] = {"J": Jm, "grad": gm, "forward_s": t_fwd, "gradient_s": t_adj, "forward_steps": fs,
+                     "device_bytes": s.device_bytes()}
+        del s
+    res["iso_256_mixed_adjoint"] = {
+        "steps": Tm, "dt": dt_m, "snapshots": 40,
+        "grad_rel_l2_mixed_vs_f64": _rel(out["mixed"]["grad"], out["f64"]["grad"]),
+        "J_rel_err_mixed_vs_f64": abs(out["mixed"]["J"] - out["f64"]["J"]) / out["f64"]["J"],
+        "gradient_over_forward_mixed": out["mixed"]["gradient_s"] / out["mixed"]["forward_s"],
+        "gradient_over_forward_f64": out["f64"]["gradient_s"] / out["f64"]["forward_s"],
+        "forward_steps_in_gradient": out["mixed"]["forward_steps"],
+        "device_bytes_mixed": out["mixed"]["device_bytes"], "device_bytes_f64": out["f64"]["device_bytes"]}
+    print(f"[bench] iso 256 mixed adjoint: {res['iso_256_mixed_adjoint']}", file=sys.stderr, flush=True)
+    return res
 
 
 def main():
@@ -586,6 +717,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-traffic", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-configs", action="store_true", help="skip the accuracy/adjoint configuration block")
     ap.add_argument("--workload", default="step",
                     choices=["step", "mixed-error", "adjoint-tubes", "adjoint-iso-mixed", "hbm-ceiling"],
                     help="step: the throughput line (default); the others run the accuracy/adjoint configurations "
@@ -615,6 +747,7 @@ def main():
     achieved = pb[dom] / (avg[dom] * 1e-3) / 1e9
     traffic = None if args.no_traffic else measure_traffic(n, dom)
     adj = adjoint_cost(n) if not args.no_adjoint else None
+    cfg = None if args.no_configs else config_evidence(n)
     cpu = None if args.no_cpu else cpu_baseline(n)
     sb = step_bytes(n, 8)
     pass_gbps = {k: pb[k] / (avg[k] * 1e-3) / 1e9 for k in avg}
@@ -649,6 +782,7 @@ def main():
                   "steps_compared": args.warmup + args.steps + timing["f64"]["steps"]},
         "fft_c2r": fft,
         "adjoint": adj,
+        "configs": cfg,
     }
     print(json.dumps(line), flush=True)
 
