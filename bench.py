@@ -269,14 +269,15 @@ def workload_mixed_error(sizes=(256, 512), nsteps: int = 1000) -> dict:
 
 def workload_hbm_ceiling() -> dict:
     """Practical HBM ceilings on this device for the passes' access patterns
-    (vonkarman_amd/lib/hbm_probe, built with the library): a contiguous copy
+    (vonkarman_amd/lib/hbm_probe, built by `make -C vonkarman_amd/csrc probe`):
+    contiguous copies with one and several loads in flight per thread
     and tiles of T adjacent 16-byte columns at 384 positions, the pattern of
     the strided x and y passes (T = 8 there, 128-byte rows)."""
     import subprocess
 
     exe = os.path.join(ROOT, "vonkarman_amd", "lib", "hbm_probe")
     if not os.path.exists(exe):
-        raise SystemExit(f"bench.py: {exe} is missing: build with make -C vonkarman_amd/csrc")
+        raise SystemExit(f"bench.py: {exe} is missing: build it with make -C vonkarman_amd/csrc probe")
     cp = subprocess.run([exe], stdout=subprocess.PIPE, stderr=subprocess.PIPE, timeout=300, check=True)
     rows = [json.loads(l) for l in cp.stdout.decode().splitlines() if l.startswith("{")]
     return {"workload": "HBM ceilings of the pass access patterns (2 GiB buffers, 10 timed launches each)",

@@ -22,11 +22,38 @@ struct d2 {
     double a, b;
 };
 
-// grid-stride copy of n16 16-byte elements
+// grid-stride copy of n16 16-byte elements, one load in flight per thread
 __global__ void k_copy(const d2 *__restrict__ in, d2 *__restrict__ out, long long n16) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long st = (long long)gridDim.x * blockDim.x;
     for (; i < n16; i += st) out[i] = in[i];
+}
+
+// copy with U loads in flight per thread: block b handles elements
+// [b U 256, (b+1) U 256), element k of thread t at b U 256 + k 256 + t (each
+// load instruction of a wave covers 1 KB contiguous); all U loads are issued
+// before the first store.  NTMP: non-temporal loads and stores.
+template <int U, bool NTMP>
+__global__ __launch_bounds__(256) void k_copy_u(const d2 *__restrict__ in, d2 *__restrict__ out, long long n16) {
+    using V = double __attribute__((ext_vector_type(2)));
+    const long long base = (long long)blockIdx.x * (U * 256) + threadIdx.x;
+    V v[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+        const long long i = base + k * 256;
+        const long long c = i < n16 ? i : n16 - 1;
+        const V *src = reinterpret_cast<const V *>(in) + c;
+        v[k] = NTMP ? __builtin_nontemporal_load(src) : *src;
+    }
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+        const long long i = base + k * 256;
+        if (i < n16) {
+            V *dst = reinterpret_cast<V *>(out) + i;
+            if (NTMP) __builtin_nontemporal_store(v[k], dst);
+            else *dst = v[k];
+        }
+    }
 }
 
 // one block per tile of T columns; positions p = 0..P-1 of column c live at
@@ -70,6 +97,13 @@ int main() {
     };
     timeit("contiguous copy, 1024 x 256 threads", 2.0 * n16 * 16,
            [&] { hipLaunchKernelGGL(k_copy, dim3(1024), dim3(256), 0, 0, a, b, n16); });
+    const unsigned g4 = (unsigned)((n16 + 4 * 256 - 1) / (4 * 256)), g8 = (unsigned)((n16 + 8 * 256 - 1) / (8 * 256));
+    timeit("contiguous copy, 4 loads in flight per thread", 2.0 * n16 * 16,
+           [&] { hipLaunchKernelGGL((k_copy_u<4, false>), dim3(g4), dim3(256), 0, 0, a, b, n16); });
+    timeit("contiguous copy, 8 loads in flight per thread", 2.0 * n16 * 16,
+           [&] { hipLaunchKernelGGL((k_copy_u<8, false>), dim3(g8), dim3(256), 0, 0, a, b, n16); });
+    timeit("contiguous copy, 8 loads in flight per thread, non-temporal", 2.0 * n16 * 16,
+           [&] { hipLaunchKernelGGL((k_copy_u<8, true>), dim3(g8), dim3(256), 0, 0, a, b, n16); });
     // P = 384 positions (the padded length at 256^3); pitch = columns per
     // position row, a multiple of 32 so every tile is whole
     constexpr int P = 384;

@@ -20,58 +20,25 @@ namespace vk {
 
 
 // Minimum waves per SIMD asked of the compiler for each kernel class and work
-// type (it caps the VGPRs per thread: 512 / waves).
+// type (it caps the VGPRs per thread at 512 / waves; 1 = no floor).  Measured
+// at 384 (256^3 solver):
+//   * FP32 z pass: a floor of 3 lets the compiler fit the pipelined loads in
+//     128 VGPRs without scratch, i.e. 4 waves per SIMD (a floor of 4 spills
+//     16 B per lane; no floor takes ~200 VGPRs and runs 1.47x slower);
+//   * FP64 z VJP pass: a floor of 2 spills 172 B per lane to scratch, yet the
+//     reverse step drops 24.3 -> 22.9 ms (unconstrained: 256 VGPRs + 38 AGPRs,
+//     one wave per SIMD);
+//   * FP64 x/y VJP passes: no floor (256 VGPRs + up to 176 AGPRs); a floor of
+//     2 spills ~600 B per lane and is slower (22.8 -> 23.2 ms);
+//   * FP32 z VJP pass: no floor (174 VGPRs, two waves); a floor of 3 (44 B
+//     spilled) measured the same;
+//   * FP64 forward/stage passes: a floor of 2 keeps the stage-3/4 epilogues
+//     at two waves (no AGPR overflow).
 enum KernelClass { K_INV, K_FWD, K_FVJP, K_ZNL, K_ZVJP };
-#ifndef VK_WAVES_INV_F32
-#define VK_WAVES_INV_F32 1
-#endif
-#ifndef VK_WAVES_FWD_F32
-#define VK_WAVES_FWD_F32 1
-#endif
-#ifndef VK_WAVES_ZNL_F32
-// FP32 z pass: a floor of 3 waves per SIMD.  The compiler then fits the
-// pipelined loads in 128 VGPRs without scratch, which is 4 waves per SIMD
-// (a floor of 4 spills 16 B per lane; no floor takes ~200 VGPRs, 1.47x slower)
-#define VK_WAVES_ZNL_F32 3
-#endif
-#ifndef VK_WAVES_ZNL_F64
-#define VK_WAVES_ZNL_F64 1
-#endif
-#ifndef VK_WAVES_INV_F64
-#define VK_WAVES_INV_F64 1
-#endif
-// FP64 z VJP pass: a 2-wave floor.  Unconstrained it takes 256 VGPRs + 38
-// AGPRs (one wave per SIMD); at two waves it spills 172 B per lane to scratch
-// and still runs the reverse step 24.3 -> 22.9 ms at 256^3.
-#ifndef VK_WAVES_ZVJP_F64
-#define VK_WAVES_ZVJP_F64 2
-#endif
-// FP64 x/y VJP passes: no floor (256 VGPRs + up to 176 AGPRs, one wave per
-// SIMD).  A 2-wave floor spills ~600 B per lane: reverse step 22.8 -> 23.2 ms.
-// FP32 z VJP pass: no floor (174 VGPRs, two waves per SIMD); a 3-wave floor
-// (168 VGPRs, 44 B spilled per lane) measured the same.
-#ifndef VK_WAVES_ZVJP_F32
-#define VK_WAVES_ZVJP_F32 1
-#endif
-#ifndef VK_WAVES_FVJP_F64
-#define VK_WAVES_FVJP_F64 1
-#endif
-#ifndef VK_WAVES_FWD_F64
-#define VK_WAVES_FWD_F64 2  // 256 VGPRs: keeps the stage-3/4 epilogues at 2 waves (no AGPR overflow)
-#endif
 template <int K, typename TW>
 struct Waves {
-    static constexpr int value = sizeof(TW) == 8 ? (K == K_ZNL    ? VK_WAVES_ZNL_F64
-                                                    : K == K_INV  ? VK_WAVES_INV_F64
-                                                    : K == K_FWD  ? VK_WAVES_FWD_F64
-                                                    : K == K_ZVJP ? VK_WAVES_ZVJP_F64
-                                                    : K == K_FVJP ? VK_WAVES_FVJP_F64
-                                                                  : 1)
-                                 : K == K_INV  ? VK_WAVES_INV_F32
-                                 : K == K_FWD  ? VK_WAVES_FWD_F32
-                                 : K == K_ZNL  ? VK_WAVES_ZNL_F32
-                                 : K == K_ZVJP ? VK_WAVES_ZVJP_F32
-                                               : 1;
+    static constexpr int value = sizeof(TW) == 8 ? (K == K_FWD ? 2 : K == K_ZVJP ? 2 : 1)
+                                                 : (K == K_ZNL ? 3 : 1);
 };
 
 __device__ __forceinline__ long long col_addr(int col, int inner, long long os) {
@@ -88,12 +55,9 @@ __device__ __forceinline__ long long col_addr(int col, int inner, long long os) 
 // outstanding-memory counts conservatively, and the next field's prefetched
 // loads are then waited for together with every store of this field (loads
 // and stores share one in-order counter).
-#ifndef VK_STORE_PAD
-#define VK_STORE_PAD 1
-#endif
 template <typename TW, int T>
 constexpr bool pad_stores() {
-    return VK_STORE_PAD && (VK_HP_ALIGN_BYTES / (2 * (int)sizeof(TW))) % T == 0;
+    return (VK_HP_ALIGN_BYTES / (2 * (int)sizeof(TW))) % T == 0;
 }
 
 // ===========================================================================
@@ -119,24 +83,18 @@ struct InvPlan<L, INV_U3, TW> {
 };
 // the FP32 y inverse runs the 24-element plan too (4-wave blocks of 16
 // columns; 0.362 -> 0.339 ms at 384), and prefetches like the 12-element one
-#ifndef VK_F32_YINV_E24
-#define VK_F32_YINV_E24 1
-#endif
 template <int L, typename TW>
 struct InvPlan<L, INV_UOMEGA, TW> {
-    using type = std::conditional_t<sizeof(TW) == 4 && VK_F32_YINV_E24, PlanS<L>, typename PlanSmall<L>::type>;
+    using type = std::conditional_t<sizeof(TW) == 4, PlanS<L>, typename PlanSmall<L>::type>;
 };
 // The stage passes' x forward plan (k_fwd FWD_STAGE and the fused pass).  FP64
 // uses the 12-element plan (2 waves per SIMD at 256 VGPRs); FP32 uses the
 // low-pass-count plan (24 elements, radices 8 8 6 at 384): its 16-column
 // tiles then make 4-wave blocks instead of 8-wave ones, so two blocks share a
 // CU, and one LDS exchange fewer per transform.
-#ifndef VK_F32_STAGE_E24
-#define VK_F32_STAGE_E24 1
-#endif
 template <int L, typename TW>
 struct StagePlan {
-    using type = std::conditional_t<sizeof(TW) == 4 && VK_F32_STAGE_E24, PlanS<L>, PlanE12<L>>;
+    using type = std::conditional_t<sizeof(TW) == 4, PlanS<L>, PlanE12<L>>;
 };
 // The padded x inverse runs the stage plan in reversed radix order, the
 // order of the fused stage pass's inverse (k_fwd_stage_inv), so the fused and
@@ -158,21 +116,12 @@ template <int L, typename TW>
 struct InvPlan<L, INV_U3DX, TW> {
     using type = typename XPassPlan<L, TW>::type;
 };
-// bytes of each position row of a strided tile (T = row bytes / element bytes columns)
-#ifndef VK_ROW_BYTES_INV
-#define VK_ROW_BYTES_INV 128
-#endif
-#ifndef VK_ROW_BYTES_FWD
-#define VK_ROW_BYTES_FWD 128
-#endif
-#ifndef VK_ROW_BYTES_INV_F32
-#define VK_ROW_BYTES_INV_F32 VK_ROW_BYTES_INV
-#endif
-#ifndef VK_ROW_BYTES_FWD_F32
-#define VK_ROW_BYTES_FWD_F32 VK_ROW_BYTES_FWD
-#endif
+// bytes of each position row of a strided tile (T = row bytes / element bytes
+// columns): 128 in every strided pass.  Wider rows need tiles of 98 KB (one
+// block per CU) and measured slower.
+constexpr int kRowBytes = 128;
 template <int L, int MODE, typename TW>
-using InvGeom = Geom<L, InvPlan<L, MODE, TW>::type::E, TW, sizeof(TW) == 8 ? VK_ROW_BYTES_INV : VK_ROW_BYTES_INV_F32>;
+using InvGeom = Geom<L, InvPlan<L, MODE, TW>::type::E, TW, kRowBytes>;
 
 template <int L, int LIN, int MODE, typename TI, typename TW>
 __global__ __launch_bounds__((InvGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_waves_per_eu(Waves<K_INV, TW>::value))) void k_inv(const cpx<TI> *__restrict__ in,
@@ -344,36 +293,22 @@ __global__ __launch_bounds__((InvGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
 // in xo; it is stored to sa.out only when sa.out is set (uniform branch).
 // k2[i]: |k|^2 of mode i in FP64 (kx^2 + ky^2 + kz^2, as SpectralOps::k_mag_sq),
 // used by the RK4 stages' viscous term.
-// the stage passes stream their input fields and the fused pass its
-// x-inverse outputs (non-temporal), so the ETD table stays cached
-#ifndef VK_NT_FUSED
-#define VK_NT_FUSED 1
-#endif
-// Streaming (non-temporal) accesses of the state arrays in the stage
-// epilogues: each element is read or written once per pass, and the ETD
-// table read beside them stays cached.
-#ifndef VK_NT_STATE
-#define VK_NT_STATE 1
-#endif
+// The stage passes stream (non-temporal loads and stores) their input
+// fields, the state arrays of the epilogues and the fused pass's x-inverse
+// outputs: each element is touched once per pass, and streaming keeps the
+// ETD table's lines cached (fused pass 0.70 -> 0.66 ms, stage 4 0.35 -> 0.33
+// ms at 384 FP64).
 template <typename R>
 __device__ __forceinline__ cpx<R> ldnt(const cpx<R> *p) {
     using V = R __attribute__((ext_vector_type(2)));
-    if constexpr (VK_NT_STATE) {
-        const V v = __builtin_nontemporal_load(reinterpret_cast<const V *>(p));
-        return {v.x, v.y};
-    } else {
-        return *p;
-    }
+    const V v = __builtin_nontemporal_load(reinterpret_cast<const V *>(p));
+    return {v.x, v.y};
 }
 template <typename R>
 __device__ __forceinline__ void stnt(cpx<R> *p, cpx<R> v) {
     using V = R __attribute__((ext_vector_type(2)));
-    if constexpr (VK_NT_STATE) {
-        const V x = {v.re, v.im};
-        __builtin_nontemporal_store(x, reinterpret_cast<V *>(p));
-    } else {
-        *p = v;
-    }
+    const V x = {v.re, v.im};
+    __builtin_nontemporal_store(x, reinterpret_cast<V *>(p));
 }
 template <int STG, int CH, typename TS, bool XO = false>
 __device__ __forceinline__ void stage_chunk(const StageArgs &sa, bool st, const long long (&idx)[CH],
@@ -679,7 +614,7 @@ struct FwdPlan<L, FWD_VJP, TW> {
     using type = PlanE12<L>;
 };
 template <int L, int MODE, typename TW>
-using FwdGeom = Geom<L, FwdPlan<L, MODE, TW>::type::E, TW, sizeof(TW) == 8 ? VK_ROW_BYTES_FWD : VK_ROW_BYTES_FWD_F32>;
+using FwdGeom = Geom<L, FwdPlan<L, MODE, TW>::type::E, TW, kRowBytes>;
 
 template <int L, int LOUT, int MODE, int NF, int STG, typename TW>
 __global__ __launch_bounds__((FwdGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_waves_per_eu(Waves<MODE == FWD_VJP ? K_FVJP : K_FWD, TW>::value))) void k_fwd(const cpx<TW> *__restrict__ in,
@@ -724,7 +659,7 @@ __global__ __launch_bounds__((FwdGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const int p = ci_pos_in<L, PL::first, T, NT>(tid, e);
-            S[e] = (MODE == FWD_STAGE && VK_NT_FUSED) ? ldg_nt(src, (long long)p * g.in_ls) : ldg(src, (long long)p * g.in_ls);
+            S[e] = MODE == FWD_STAGE ? ldg_nt(src, (long long)p * g.in_ls) : ldg(src, (long long)p * g.in_ls);
         }
     };
     auto load_field = [&](int f) { load_into(v, f); };
@@ -912,16 +847,13 @@ __global__ __launch_bounds__((FwdGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
 // mode) runs the 12-element plan: with the 24-element one its registers spill
 // into AGPRs (256 VGPRs + 54 AGPRs, one wave per SIMD); with 12 elements it
 // needs 228 VGPRs and runs 8-wave blocks at two waves per SIMD.
-#ifndef VK_F32_ST3_E12
-#define VK_F32_ST3_E12 1
-#endif
 template <int L, typename TW, int STG>
 struct FusedPlan {
-    using type = std::conditional_t<sizeof(TW) == 4 && VK_F32_ST3_E12 && STG == ST_3, PlanE12<L>,
+    using type = std::conditional_t<sizeof(TW) == 4 && STG == ST_3, PlanE12<L>,
                                     typename StagePlan<L, TW>::type>;
 };
 template <int L, int STG, typename TW>
-using FusedGeom = Geom<L, FusedPlan<L, TW, STG>::type::E, TW, sizeof(TW) == 8 ? VK_ROW_BYTES_FWD : VK_ROW_BYTES_FWD_F32>;
+using FusedGeom = Geom<L, FusedPlan<L, TW, STG>::type::E, TW, kRowBytes>;
 
 template <int L, int LOUT, int STG, typename TW>
 __global__ __launch_bounds__((FusedGeom<L, STG, TW>::NT)) __attribute__((amdgpu_waves_per_eu(Waves<K_FWD, TW>::value))) void k_fwd_stage_inv(
@@ -939,10 +871,7 @@ __global__ __launch_bounds__((FusedGeom<L, STG, TW>::NT)) __attribute__((amdgpu_
     constexpr bool HEAVY = STG == ST_3;
     // kept modes per epilogue chunk.  FP64 takes one: two spilled 40 B per
     // lane in stages 1-2 (fused pass 0.663 -> 0.648 ms with one, same box).
-#ifndef VK_FUSED_CH_F64
-#define VK_FUSED_CH_F64 1
-#endif
-    constexpr int CH = sizeof(TW) == 8 ? (HEAVY ? 1 : VK_FUSED_CH_F64) : (HEAVY ? 2 : 4);
+    constexpr int CH = sizeof(TW) == 8 ? 1 : (HEAVY ? 2 : 4);
     static_assert(K % CH == 0, "chunks must tile the kept modes");
     __shared__ cpx<TW> lds[L * T];
     __shared__ cpx<TW> ltw[L];
@@ -963,7 +892,7 @@ __global__ __launch_bounds__((FusedGeom<L, STG, TW>::NT)) __attribute__((amdgpu_
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const long long i = (long long)ci_pos_in<L, PL::first, T, NT>(tid, e) * g.in_ls;
-            S[e] = VK_NT_FUSED ? ldg_nt(src, i) : ldg(src, i);
+            S[e] = ldg_nt(src, i);
         }
     };
     // forward transforms of the three fields (each next field loads meanwhile)
@@ -1040,8 +969,7 @@ __global__ __launch_bounds__((FusedGeom<L, STG, TW>::NT)) __attribute__((amdgpu_
             for (int e = 0; e < E; ++e)
             {
                 cpx<TW> *dst = aout + fo * gi.out_fs + cinv + (long long)ci_pos_out<L, RL::last, T, NT>(tid, e) * gi.out_ls;
-                if (VK_NT_FUSED) stg_nt(dst, v[e]);
-                else *dst = v[e];
+                stg_nt(dst, v[e]);
             }
         }
         __syncthreads();
@@ -1187,36 +1115,19 @@ __device__ __forceinline__ void z_pair_fill(cpx<TW> (&v)[E], gptr<cpx<TW>> fa, g
 // HBM in runs of 32 consecutive positions per load instruction.  Measured at
 // 384 (256^3 solver), per launch: FP64 1.03 ms with 4 lines per block -> 0.87
 // ms with 2; FP32 0.65 ms with 4 -> 0.50 ms with 2.
-#ifndef VK_Z_ROW_BYTES
-#define VK_Z_ROW_BYTES 32  // T = 2 FP64 lines per block
-#endif
-#ifndef VK_Z_ROW_BYTES_F32
-#define VK_Z_ROW_BYTES_F32 16  // T = 2 FP32 lines per block
-#endif
+//
 // z passes read their twiddles from a per-block LDS copy: a global twiddle
 // load inside a radix pass is a dependent memory wait that a one-wave block
-// cannot hide
-#ifndef VK_Z_LDS_TW
-#define VK_Z_LDS_TW 1
-#endif
-// the nonlinear z pass issues each next input pair's loads before the current
-// pair's transform (at 384: FP64 0.738 -> 0.704 ms; FP32 0.395 -> 0.387 ms,
-// at 128 VGPRs under its 3-wave floor)
-#ifndef VK_Z_PIPE_F64
-#define VK_Z_PIPE_F64 1
-#endif
-#ifndef VK_Z_PIPE_F32
-#define VK_Z_PIPE_F32 1
-#endif
+// cannot hide.
 template <int L, int D, int T, int NT, typename PL, int E, typename R>
-__device__ __forceinline__ void zfft(cpx<R> (&v)[E], cpx<R> *lds, const cpx<R> *ltw, const cpx<R> *tw, int tid) {
-    if constexpr (VK_Z_LDS_TW) fft_reg_lt<L, D, T, NT, T, 1, PL>(v, lds, ltw, tid);
-    else fft_reg<L, D, T, NT, T, 1, PL>(v, lds, tw, tid);
+__device__ __forceinline__ void zfft(cpx<R> (&v)[E], cpx<R> *lds, const cpx<R> *ltw, int tid) {
+    fft_reg_lt<L, D, T, NT, T, 1, PL>(v, lds, ltw, tid);
 }
 template <int L, typename TW>
 using ZNLP = PlanE12<L>;
+// T = 2 lines per block: rows of 32 B (FP64) or 16 B (FP32)
 template <int L, typename TW>
-using ZNLGeom = Geom<L, ZNLP<L, TW>::E, TW, sizeof(TW) == 8 ? VK_Z_ROW_BYTES : VK_Z_ROW_BYTES_F32>;
+using ZNLGeom = Geom<L, ZNLP<L, TW>::E, TW, 2 * 2 * (int)sizeof(TW)>;
 
 template <int L, typename TW>
 __global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) __attribute__((amdgpu_waves_per_eu(Waves<K_ZNL, TW>::value))) void k_z_nl(const cpx<TW> *__restrict__ in,
@@ -1230,10 +1141,10 @@ __global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) __attribute__((amdgpu_waves_p
     constexpr int T = ZNLGeom<L, TW>::T, NT = ZNLGeom<L, TW>::NT;
     constexpr int HIN = L / 3 + 1;  // N/2+1 with N = 2L/3
     __shared__ cpx<TW> lds[L * T];
-    __shared__ cpx<TW> ltw[VK_Z_LDS_TW ? L : 1];
+    __shared__ cpx<TW> ltw[L];
     const int tid = threadIdx.x;
     const long long line0 = (long long)blockIdx.x * T;
-    if constexpr (VK_Z_LDS_TW) stage_twiddles<L, NT>(ltw, tw, tid);
+    stage_twiddles<L, NT>(ltw, tw, tid);
     const int t = tid % T;
     const bool ok = line0 + t < lines;
     // lines past the end load the last line (valid memory); their results are not stored
@@ -1242,24 +1153,20 @@ __global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) __attribute__((amdgpu_waves_p
     TW r[6][E];
 
     // fields: 0 ux, 1 uy, 2 uz, 3 wx, 4 wy, 5 wz; pairs (0,1) (2,3) (4,5).
-    // Each next pair's loads are in flight during the current pair's transform.
-    constexpr bool PIPE = sizeof(TW) == 8 ? VK_Z_PIPE_F64 : VK_Z_PIPE_F32;
+    // Each next pair's loads are in flight during the current pair's
+    // transform (at 384: FP64 0.738 -> 0.704 ms; FP32 0.395 -> 0.387 ms, at
+    // 128 VGPRs under its 3-wave floor).
     ZRaw<TW> raw[E];
     auto issue = [&](int q) {
         z_pair_issue<L, HIN, PL::first, T, NT, true>(raw, opaque(in) + (2 * q) * fs_in,
                                                       opaque(in) + (2 * q + 1) * fs_in, base, tid);
     };
-    if (PIPE) issue(0);
+    issue(0);
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-        if (PIPE) {
-            z_pair_combine<L, HIN, PL::first, T, NT, true>(v, raw, tid);
-            if (q + 1 < 3) issue(q + 1);
-        } else {
-            const gptr<cpx<TW>> fa = opaque(in) + (2 * q) * fs_in, fb = opaque(in) + (2 * q + 1) * fs_in;
-            z_pair_fill<L, HIN, PL::first, T, NT, true>(v, fa, fb, base, tid);
-        }
-        zfft<L, +1, T, NT, PL>(v, lds, ltw, tw, tid);
+        z_pair_combine<L, HIN, PL::first, T, NT, true>(v, raw, tid);
+        if (q + 1 < 3) issue(q + 1);
+        zfft<L, +1, T, NT, PL>(v, lds, ltw, tid);
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             r[2 * q][e] = v[e].re * s_in;
@@ -1280,7 +1187,7 @@ __global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) __attribute__((amdgpu_waves_p
     // (c0 + i c1): forward, then split through LDS (needs k and L-k)
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = {r[0][e], r[1][e]};
-    zfft<L, -1, T, NT, RL>(v, lds, ltw, tw, tid);
+    zfft<L, -1, T, NT, RL>(v, lds, ltw, tid);
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < E; ++e) lds[ci_pos_out<L, RL::last, T, NT>(tid, e) * T + t] = v[e];
@@ -1299,7 +1206,7 @@ __global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) __attribute__((amdgpu_waves_p
     // c2 alone: its first HOUT outputs are stored from registers
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = {r[2][e], TW(0)};
-    zfft<L, -1, T, NT, RL>(v, lds, ltw, tw, tid);
+    zfft<L, -1, T, NT, RL>(v, lds, ltw, tid);
     if (ok) {
 #pragma unroll
         for (int e = 0; e < E; ++e) {
@@ -1348,10 +1255,10 @@ __global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) __attribute__((amdgpu_waves_p
     constexpr int T = ZNLGeom<L, TW>::T, NT = ZNLGeom<L, TW>::NT;
     constexpr int HIN = L / 3 + 1, HOUT = HIN;
     __shared__ cpx<TW> lds[L * T];
-    __shared__ cpx<TW> ltw[VK_Z_LDS_TW ? L : 1];
+    __shared__ cpx<TW> ltw[L];
     const int tid = threadIdx.x;
     const long long line0 = (long long)blockIdx.x * T;
-    if constexpr (VK_Z_LDS_TW) stage_twiddles<L, NT>(ltw, tw, tid);
+    stage_twiddles<L, NT>(ltw, tw, tid);
     const int t = tid % T;
     const bool ok = line0 + t < lines;
     // lines past the end load the last line (valid memory); their results are not stored
@@ -1369,7 +1276,7 @@ __global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) __attribute__((amdgpu_waves_p
         else z_pair_fill<L, HIN, PL::first, T, NT, false>(v, pa, pa, base, tid);
         // position 0 is element 0 of thread column 0's first row
         if (ci_pos_in<L, PL::first, T, NT>(tid, 0) == 0) v[0] = {v[0].re * dca, v[0].im * dcb};
-        zfft<L, +1, T, NT, PL>(v, lds, ltw, tw, tid);
+        zfft<L, +1, T, NT, PL>(v, lds, ltw, tid);
     };
     // Cotangent fields are paired only with cotangent fields: a complex FFT
     // of (a + i b) leaks roundoff of the larger of a, b into the other, and u, w
@@ -1405,7 +1312,7 @@ __global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) __attribute__((amdgpu_waves_p
         v[e] = {gw0, gw1};
     }
     __syncthreads();
-    zfft<L, -1, T, NT, RL>(v, lds, ltw, tw, tid);
+    zfft<L, -1, T, NT, RL>(v, lds, ltw, tid);
     __syncthreads();
     z_split_store<L, T, NT, RL, E, HOUT>(v, lds, tid, line0, lines, hp, out + 3 * fs_out, out + 4 * fs_out, s_o);
     // live: g0, g1, g2 = r0, gU1 part = r1, gU2 part = r2, gW2 = r3
@@ -1419,12 +1326,12 @@ __global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) __attribute__((amdgpu_waves_p
         v[e] = {gu0, gu1};
     }
     __syncthreads();
-    zfft<L, -1, T, NT, RL>(v, lds, ltw, tw, tid);
+    zfft<L, -1, T, NT, RL>(v, lds, ltw, tid);
     __syncthreads();
     z_split_store<L, T, NT, RL, E, HOUT>(v, lds, tid, line0, lines, hp, out + 0 * fs_out, out + 1 * fs_out, s_o);
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = {r0[e], r3[e]};
-    zfft<L, -1, T, NT, RL>(v, lds, ltw, tw, tid);
+    zfft<L, -1, T, NT, RL>(v, lds, ltw, tid);
     __syncthreads();
     z_split_store<L, T, NT, RL, E, HOUT>(v, lds, tid, line0, lines, hp, out + 2 * fs_out, out + 5 * fs_out, s_o);
 }

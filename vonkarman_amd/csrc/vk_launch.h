@@ -6,9 +6,7 @@
 
 // Spectral rows (the kz axis, h = n/2 + 1 modes) are padded to hp, a multiple
 // of VK_HP_ALIGN_BYTES / element bytes, so strided tiles cover whole rows.
-#ifndef VK_HP_ALIGN_BYTES
-#define VK_HP_ALIGN_BYTES 128
-#endif
+constexpr int VK_HP_ALIGN_BYTES = 128;
 
 namespace vk {
 
