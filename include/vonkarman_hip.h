@@ -33,8 +33,11 @@ extern "C" {
 #define VK_ERR_NOMEM 4       /* device allocation failed */
 
 #define VK_PREC_F64 0   /* all arithmetic in FP64 */
-#define VK_PREC_MIXED 1 /* FFTs, u x w and projection in FP32; state, phi
-                           table, twiddle generation and reductions in FP64 */
+#define VK_PREC_MIXED 1 /* FFTs, u x w, projection and the state and stage
+                           arrays in FP32; phi table, stage combines (FP64
+                           arithmetic rounded once to FP32), twiddle
+                           generation and reductions in FP64.  Host state
+                           buffers stay complex double at the boundary. */
 
 /* Thread-local description of the last error ("" if none). */
 const char *vk_last_error(void);

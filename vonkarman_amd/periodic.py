@@ -123,8 +123,10 @@ class Periodic3D:
     ``Periodic3D(grid, nu, ic)`` follows Periodic3D::new (solver.rs:38-108):
     the initial field is transformed to spectral space and dt is set from the
     CFL rule (safety 0.5).  ``set_fixed_dt(dt)`` selects the fixed-dt mode
-    the reference lacks.  ``precision="mixed"`` runs the FFTs, u x w and the
-    projection in FP32 with the state, the phi table and the reductions in FP64.
+    the reference lacks.  ``precision="mixed"`` runs the FFTs, u x w, the
+    projection and the state and stage arrays in FP32, and keeps the phi table,
+    the stage-combine arithmetic (rounded once to FP32) and the reductions in
+    FP64; states cross the boundary as complex128.
     """
 
     def __init__(self, grid: GridSpec, nu: float, ic=None, backend_mode: BackendMode = BackendMode.Hip,
