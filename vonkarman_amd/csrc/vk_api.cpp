@@ -180,7 +180,7 @@ static int plan_c2r_kernels(vk_fft_plan *p, const void *spec) {
                           p->tw_x.p, p->tw_y.p, p->tw_z.p, p->st);
 }
 
-static int plan_r2c_kernels(vk_fft_plan *p) {
+static int plan_r2c_kernels(vk_fft_plan *p, void *spec) {
     const int nx = p->nx, ny = p->ny, nz = p->nz, hp = p->hp, hz = p->hz;
     // z R2C: real -> t1 [ix][iy][hp]
     VK_TRY(launch_z_r2c(nz, F64, static_cast<const double *>(p->real.p), p->t1.p, (long long)nx * ny, hp, p->tw_z.p,
@@ -188,9 +188,11 @@ static int plan_r2c_kernels(vk_fft_plan *p) {
     // y forward: t1 [ix][iy][hp] -> t2 [iy][ix][hp]
     Strided gy = make_strided(nx * hp, hp, hz, hp, (long long)ny * hp, hp, 0, nx * hp, 0, (long long)nx * hp, 0);
     VK_TRY(launch_fwd(FWD_PLAIN, ny, ny, 1, F64, p->t1.p, p->t2.p, p->tw_y.p, gy, ny, 1.0, 1.0, nullptr, p->st));
-    // x forward: t2 [iy][ix][hp] -> t1 natural [ix][iy][hp]
-    Strided gx = make_strided(ny * hp, hp, hz, hp, (long long)nx * hp, hp, 0, ny * hp, 0, (long long)ny * hp, 0);
-    VK_TRY(launch_fwd(FWD_PLAIN, nx, nx, 1, F64, p->t2.p, p->t1.p, p->tw_x.p, gx, nx, 1.0, 1.0, nullptr, p->st));
+    // x forward: t2 [iy][ix][hp] -> the caller's layout [ix][iy][hz]: column
+    // (iy, iz) of the padded column space stores at iy*hz + iz, valid
+    // columns only
+    Strided gx = make_strided(ny * hp, hp, hz, hp, (long long)nx * hp, hp, 0, hp, hz, (long long)ny * hz, 0);
+    VK_TRY(launch_fwd(FWD_PLAIN, nx, nx, 1, F64, p->t2.p, spec, p->tw_x.p, gx, nx, 1.0, 1.0, nullptr, p->st));
     return 0;
 }
 
@@ -289,8 +291,7 @@ static int fft_r2c_impl(vk_fft_plan *p, const double *in, double *out, bool devi
     VK_HIP(hipEventRecord(p->ev0, p->st));
     VK_HIP(hipMemcpyAsync(p->real.p, in, real_bytes, device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, p->st));
     VK_HIP(hipEventRecord(p->ev1, p->st));
-    if (int r = plan_r2c_kernels(p)) return r;
-    VK_TRY(launch_repack(p->t1.p, p->spec.p, (long long)p->nx * p->ny, p->hp, p->hz, p->hz, 1, p->st));
+    if (int r = plan_r2c_kernels(p, p->spec.p)) return r;
     VK_HIP(hipEventRecord(p->ev2, p->st));
     VK_HIP(hipMemcpyAsync(out, p->spec.p, spec_bytes, device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, p->st));
     VK_HIP(hipEventRecord(p->ev3, p->st));

@@ -32,13 +32,14 @@ namespace vk {
 //     2 spills ~600 B per lane and is slower (22.8 -> 23.2 ms);
 //   * FP32 z VJP pass: no floor (174 VGPRs, two waves); a floor of 3 (44 B
 //     spilled) measured the same;
-//   * FP64 forward/stage passes: a floor of 2 keeps the stage-3/4 epilogues
-//     at two waves (no AGPR overflow).
+//   * forward/stage passes: a floor of 2 keeps the stage-3/4 epilogues at two
+//     waves (FP64: no AGPR overflow; FP32 stage 3 of the fused pass: 256
+//     VGPRs and 17 B spilled instead of 256 VGPRs + 19 AGPRs at one wave).
 enum KernelClass { K_INV, K_FWD, K_FVJP, K_ZNL, K_ZVJP };
 template <int K, typename TW>
 struct Waves {
     static constexpr int value = sizeof(TW) == 8 ? (K == K_FWD ? 2 : K == K_ZVJP ? 2 : 1)
-                                                 : (K == K_ZNL ? 3 : 1);
+                                                 : (K == K_ZNL ? 3 : K == K_FWD ? 2 : 1);
 };
 
 __device__ __forceinline__ long long col_addr(int col, int inner, long long os) {
@@ -807,7 +808,10 @@ __global__ __launch_bounds__((FwdGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
         // pad_stores), and whether an element's outputs are kept under the
         // truncation is decided at compile time wherever its whole position
         // range is kept or dropped: no store sits under a per-lane branch.
-        constexpr bool PADP = pad_stores<TW, T>();
+        // The one-field instance (the standalone R2C's y and x passes) stores
+        // valid columns only, so its x pass can write the caller's unpadded
+        // rows of nz/2+1 directly.
+        constexpr bool PADP = pad_stores<TW, T>() && NF != 1;
         const bool stp = PADP || colok;
         const long long coutp = stp ? col_addr(col, g.out_inner, g.out_os) : 0;
         auto store_field = [&](int f) {
@@ -843,14 +847,14 @@ __global__ __launch_bounds__((FwdGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
 // Outputs (5 fields, the x pass layout of the next nonlinear evaluation):
 // 0 ux, 1 uy, 2 uz, 3 i kx uz, 4 i kx uy.
 // ===========================================================================
-// The fused pass's plan per stage.  FP32 stage 3 (three state arrays per kept
-// mode) runs the 12-element plan: with the 24-element one its registers spill
-// into AGPRs (256 VGPRs + 54 AGPRs, one wave per SIMD); with 12 elements it
-// needs 228 VGPRs and runs 8-wave blocks at two waves per SIMD.
+// The fused pass runs the stage plan of its precision in every stage.  FP32
+// stage 3 (three state arrays per kept mode) takes one kept mode per epilogue
+// chunk so it fits two waves per SIMD: 4-wave blocks, two per CU (the
+// 12-element plan needs 226 VGPRs and 8-wave blocks, one per CU: fused pass
+// 0.4315 -> 0.4219 ms at 384, same box, the same bits).
 template <int L, typename TW, int STG>
 struct FusedPlan {
-    using type = std::conditional_t<sizeof(TW) == 4 && STG == ST_3, PlanE12<L>,
-                                    typename StagePlan<L, TW>::type>;
+    using type = typename StagePlan<L, TW>::type;
 };
 template <int L, int STG, typename TW>
 using FusedGeom = Geom<L, FusedPlan<L, TW, STG>::type::E, TW, kRowBytes>;
@@ -871,7 +875,7 @@ __global__ __launch_bounds__((FusedGeom<L, STG, TW>::NT)) __attribute__((amdgpu_
     constexpr bool HEAVY = STG == ST_3;
     // kept modes per epilogue chunk.  FP64 takes one: two spilled 40 B per
     // lane in stages 1-2 (fused pass 0.663 -> 0.648 ms with one, same box).
-    constexpr int CH = sizeof(TW) == 8 ? 1 : (HEAVY ? 2 : 4);
+    constexpr int CH = sizeof(TW) == 8 || HEAVY ? 1 : 4;
     static_assert(K % CH == 0, "chunks must tile the kept modes");
     __shared__ cpx<TW> lds[L * T];
     __shared__ cpx<TW> ltw[L];
@@ -1337,8 +1341,17 @@ __global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) __attribute__((amdgpu_waves_p
 }
 
 // C2R of three unpadded fields and the maximum of |v|^2 (CFL speed or |w|).
+// The CFL max pass: T lines of 128-B rows per block.  In one-wave blocks of 2
+// lines its single atomic per block made the 256^3 CFL pass 0.50 -> 0.76 ms.
 template <int L, typename TW>
 using ZMaxGeom = Geom<L, PlanSmall<L>::type::E, TW>;
+// The standalone C2R and R2C along the contiguous axis use the z pass's tile:
+// blocks of one wave holding T = 2 transform lines (more for short lines), the
+// smaller-E plan and twiddles in LDS, so each load or store instruction covers
+// 32 (or more) consecutive positions of each line (384^3: C2R 0.661 -> 0.567
+// ms, R2C 0.762 -> 0.707 ms for the whole 3D transform).
+template <int L, typename TW>
+using ZLineGeom = Geom<L, PlanSmall<L>::type::E, TW, 2 * 2 * (int)sizeof(TW)>;
 
 template <int L, typename TW>
 __global__ __launch_bounds__((ZMaxGeom<L, TW>::NT)) void k_z_max(const cpx<TW> *__restrict__ in, long long lines,
@@ -1401,32 +1414,40 @@ __global__ __launch_bounds__((ZMaxGeom<L, TW>::NT)) void k_z_max(const cpx<TW> *
 // standalone C2R / R2C of one field along the contiguous axis: data lines
 // (2s, 2s+1) share transform line s; a block holds T transform lines.
 template <int L, typename TW>
-using ZSGeom = Geom<L, PlanS<L>::E, TW>;
+using ZSGeom = ZLineGeom<L, TW>;
 
 template <int L, typename TW>
 __global__ __launch_bounds__((ZSGeom<L, TW>::NT)) void k_z_c2r(const cpx<TW> *__restrict__ in,
                                                                double *__restrict__ out, long long lines, int hp,
                                                                TW scale, const cpx<TW> *__restrict__ tw) {
-    using P = PlanS<L>;
+    using P = typename PlanSmall<L>::type;
     using PL = typename P::radices;
     constexpr int E = P::E;
     constexpr int T = ZSGeom<L, TW>::T, NT = ZSGeom<L, TW>::NT;
     constexpr int HIN = L / 2 + 1;
     __shared__ cpx<TW> lds[L * T];
+    __shared__ cpx<TW> ltw[L];
     const int tid = threadIdx.x;
+    stage_twiddles<L, NT>(ltw, tw, tid);
     const long long s = (long long)blockIdx.x * T + tid % T;
     const long long la = 2 * s, lb = la + 1;
-    cpx<TW> v[E], vb[E];
+    cpx<TW> v[E];
     // lines past the end load the last line (valid memory); b is zeroed past the end
     const gptr<cpx<TW>> src = opaque(in);
-    z_pair_fill<L, HIN, PL::first, T, NT, false>(v, src, src, (la < lines ? la : lines - 1) * hp, tid);
-    z_pair_fill<L, HIN, PL::first, T, NT, false>(vb, src, src, (lb < lines ? lb : lines - 1) * hp, tid);
+    {
+        ZRaw<TW> ra[E], rb[E];
+        z_pair_issue<L, HIN, PL::first, T, NT, false>(ra, src, src, (la < lines ? la : lines - 1) * hp, tid);
+        z_pair_issue<L, HIN, PL::first, T, NT, false>(rb, src, src, (lb < lines ? lb : lines - 1) * hp, tid);
+        cpx<TW> vb[E];
+        z_pair_combine<L, HIN, PL::first, T, NT, false>(v, ra, tid);
+        z_pair_combine<L, HIN, PL::first, T, NT, false>(vb, rb, tid);
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-        const cpx<TW> zb = lb < lines ? vb[e] : cpx<TW>{TW(0), TW(0)};
-        v[e] = {v[e].re - zb.im, v[e].im + zb.re};  // X_a + i X_b
+        for (int e = 0; e < E; ++e) {
+            const cpx<TW> zb = lb < lines ? vb[e] : cpx<TW>{TW(0), TW(0)};
+            v[e] = {v[e].re - zb.im, v[e].im + zb.re};  // X_a + i X_b
+        }
     }
-    fft_reg<L, +1, T, NT, T, 1, PL>(v, lds, tw, tid);
+    fft_reg_lt<L, +1, T, NT, T, 1, PL>(v, lds, ltw, tid);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int p = ci_pos_out<L, PL::last, T, NT>(tid, e);
@@ -1439,23 +1460,29 @@ template <int L, typename TW>
 __global__ __launch_bounds__((ZSGeom<L, TW>::NT)) void k_z_r2c(const double *__restrict__ in,
                                                                cpx<TW> *__restrict__ out, long long lines, int hp,
                                                                const cpx<TW> *__restrict__ tw) {
-    using P = PlanS<L>;
+    using P = typename PlanSmall<L>::type;
     using PL = typename P::radices;
     constexpr int E = P::E;
     constexpr int T = ZSGeom<L, TW>::T, NT = ZSGeom<L, TW>::NT;
     constexpr int HOUT = L / 2 + 1;
     __shared__ cpx<TW> lds[L * T];
+    __shared__ cpx<TW> ltw[L];
     const int tid = threadIdx.x;
+    stage_twiddles<L, NT>(ltw, tw, tid);
     const long long pair0 = (long long)blockIdx.x * T;
     const int t = tid % T;
     const long long la = 2 * (pair0 + t), lb = la + 1;
+    // lines past the end load the last line (valid memory) and are zeroed
+    const long long ca = (la < lines ? la : lines - 1) * L, cb = (lb < lines ? lb : lines - 1) * L;
+    const gptr<double> src = (gptr<double>)in;
     cpx<TW> v[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int p = ci_pos_in<L, PL::first, T, NT>(tid, e);
-        v[e] = {la < lines ? TW(in[la * L + p]) : TW(0), lb < lines ? TW(in[lb * L + p]) : TW(0)};
+        const double a = src[ca + p], b = src[cb + p];
+        v[e] = {la < lines ? TW(a) : TW(0), lb < lines ? TW(b) : TW(0)};
     }
-    fft_reg<L, -1, T, NT, T, 1, PL>(v, lds, tw, tid);
+    fft_reg_lt<L, -1, T, NT, T, 1, PL>(v, lds, ltw, tid);
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < E; ++e) lds[ci_pos_out<L, PL::last, T, NT>(tid, e) * T + t] = v[e];
@@ -1539,7 +1566,7 @@ template <int L, int LOUT, int MODE, int NF, int STG, typename TW>
 int run_fwd(const void *in, void *out, const void *tw, const Strided &g, int n, double ks, double scale,
             const StageArgs *sa, hipStream_t st) {
     using C = FwdGeom<L, MODE, TW>;
-    if (MODE == FWD_PLAIN && pad_stores<TW, C::T>() && g.ncols % C::T != 0) return 2;
+    if (MODE == FWD_PLAIN && NF != 1 && pad_stores<TW, C::T>() && g.ncols % C::T != 0) return 2;
     const unsigned nb = (unsigned)((g.ncols + C::T - 1) / C::T);
     StageArgs a{};
     if (sa) a = *sa;
