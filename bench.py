@@ -633,8 +633,12 @@ def config_evidence(n: int = 256) -> dict:
     from vonkarman_amd import ic as vic
 
     res = {}
-    # mixed vs FP64, 1000 steps at dt = 5e-4, nu = 1e-3
+    # mixed vs FP64, 1000 steps at dt = 5e-4, nu = 1e-3; and a 100-step sample
+    # of the 512^3 grid of the same configuration (nu = 4e-4; the 1000 steps
+    # there run under --workload mixed-error)
     res["mixed_1000_steps"] = workload_mixed_error(sizes=(n,), nsteps=1000)[str(n)]
+    if n == 256:
+        res["mixed_512_100_steps"] = workload_mixed_error(sizes=(512,), nsteps=100)["512"]
     # anti-parallel tubes at 128^3: 200 steps, dt = 2e-3, J = final
     # enstrophy, a snapshot every 20 steps; fourth-order central differences
     # (h = 1e-4 relative) along 8 seeded divergence-free directions, and the
