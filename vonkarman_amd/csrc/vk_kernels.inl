@@ -1197,17 +1197,23 @@ __global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) __attribute__((amdgpu_waves_p
     for (int e = 0; e < E; ++e) lds[ci_pos_out<L, RL::last, T, NT>(tid, e) * T + t] = v[e];
     __syncthreads();
     const TW h = TW(0.5) * s_out;
-    for (int e = tid; e < T * HOUT; e += NT) {
+    // Output lines are written whole, entries [HOUT, hp) as zeros: a line
+    // written only up to HOUT leaves its last cache line partly written,
+    // which costs HBM time (no pass reads those entries).
+    const cpx<TW> zero = {TW(0), TW(0)};
+    for (int e = tid; e < T * hp; e += NT) {
         const int tt = e % T, k = e / T;
         const long long line = line0 + tt;
-        const cpx<TW> zk = lds[k * T + tt], zm = lds[(k ? L - k : 0) * T + tt];
+        const bool kept = k < HOUT;
+        const int kk = kept ? k : 0;
+        const cpx<TW> zk = lds[kk * T + tt], zm = lds[(kk ? L - kk : 0) * T + tt];
         if (line < lines) {
-            out[0 * fs_out + line * hp + k] = {(zk.re + zm.re) * h, (zk.im - zm.im) * h};
-            out[1 * fs_out + line * hp + k] = {(zk.im + zm.im) * h, (zm.re - zk.re) * h};
+            out[0 * fs_out + line * hp + k] = kept ? cpx<TW>{(zk.re + zm.re) * h, (zk.im - zm.im) * h} : zero;
+            out[1 * fs_out + line * hp + k] = kept ? cpx<TW>{(zk.im + zm.im) * h, (zm.re - zk.re) * h} : zero;
         }
     }
     __syncthreads();
-    // c2 alone: its first HOUT outputs are stored from registers
+    // c2 alone: its first hp outputs are stored from registers
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = {r[2][e], TW(0)};
     zfft<L, -1, T, NT, RL>(v, lds, ltw, tid);
@@ -1215,7 +1221,7 @@ __global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) __attribute__((amdgpu_waves_p
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const int p = ci_pos_out<L, RL::last, T, NT>(tid, e);
-            if (p < HOUT) out[2 * fs_out + base + p] = cscale(v[e], s_out);
+            if (p < hp) out[2 * fs_out + base + p] = p < HOUT ? cscale(v[e], s_out) : zero;
         }
     }
 }
@@ -1223,22 +1229,26 @@ __global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) __attribute__((amdgpu_waves_p
 
 // split the forward DFT of (a + i b) held in registers into the truncated
 // half spectra of a and b, times c_k s (c_0 = 1, c_k = 2 otherwise): the
-// transpose of the unnormalised C2R applied to a and b.
+// transpose of the unnormalised C2R applied to a and b.  Lines are written
+// whole, entries [HOUT, hp) as zeros.
 template <int L, int T, int NT, typename RL, int E, int HOUT, typename TW>
 __device__ __forceinline__ void z_split_store(const cpx<TW> (&v)[E], cpx<TW> *lds, int tid, long long line0,
                                               long long lines, int hp, cpx<TW> *oa, cpx<TW> *ob, TW s) {
     const int t = tid % T;
+    const cpx<TW> zero = {TW(0), TW(0)};
 #pragma unroll
     for (int e = 0; e < E; ++e) lds[ci_pos_out<L, RL::last, T, NT>(tid, e) * T + t] = v[e];
     __syncthreads();
-    for (int e = tid; e < T * HOUT; e += NT) {
+    for (int e = tid; e < T * hp; e += NT) {
         const int tt = e % T, k = e / T;
         const long long line = line0 + tt;
-        const cpx<TW> zk = lds[k * T + tt], zm = lds[(k ? L - k : 0) * T + tt];
+        const bool kept = k < HOUT;
+        const int kk = kept ? k : 0;
+        const cpx<TW> zk = lds[kk * T + tt], zm = lds[(kk ? L - kk : 0) * T + tt];
         const TW h = k ? s : TW(0.5) * s;
         if (line < lines) {
-            oa[line * hp + k] = {(zk.re + zm.re) * h, (zk.im - zm.im) * h};
-            ob[line * hp + k] = {(zk.im + zm.im) * h, (zm.re - zk.re) * h};
+            oa[line * hp + k] = kept ? cpx<TW>{(zk.re + zm.re) * h, (zk.im - zm.im) * h} : zero;
+            ob[line * hp + k] = kept ? cpx<TW>{(zk.im + zm.im) * h, (zm.re - zk.re) * h} : zero;
         }
     }
     __syncthreads();
@@ -1488,12 +1498,16 @@ __global__ __launch_bounds__((ZSGeom<L, TW>::NT)) void k_z_r2c(const double *__r
     for (int e = 0; e < E; ++e) lds[ci_pos_out<L, PL::last, T, NT>(tid, e) * T + t] = v[e];
     __syncthreads();
     const TW h = TW(0.5);
-    for (int e = tid; e < T * HOUT; e += NT) {
+    const cpx<TW> zero = {TW(0), TW(0)};
+    // lines are written whole, entries [HOUT, hp) as zeros
+    for (int e = tid; e < T * hp; e += NT) {
         const int tt = e % T, k = e / T;
         const long long a = 2 * (pair0 + tt), b = a + 1;
-        const cpx<TW> zk = lds[k * T + tt], zm = lds[(k ? L - k : 0) * T + tt];
-        if (a < lines) out[a * hp + k] = {(zk.re + zm.re) * h, (zk.im - zm.im) * h};
-        if (b < lines) out[b * hp + k] = {(zk.im + zm.im) * h, (zm.re - zk.re) * h};
+        const bool kept = k < HOUT;
+        const int kk = kept ? k : 0;
+        const cpx<TW> zk = lds[kk * T + tt], zm = lds[(kk ? L - kk : 0) * T + tt];
+        if (a < lines) out[a * hp + k] = kept ? cpx<TW>{(zk.re + zm.re) * h, (zk.im - zm.im) * h} : zero;
+        if (b < lines) out[b * hp + k] = kept ? cpx<TW>{(zk.im + zm.im) * h, (zm.re - zk.re) * h} : zero;
     }
 }
 
