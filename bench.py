@@ -275,9 +275,11 @@ def workload_mixed_error(sizes=(256, 512), nsteps: int = 1000) -> dict:
 def workload_hbm_ceiling() -> dict:
     """Practical HBM ceilings on this device for the passes' access patterns
     (vonkarman_amd/lib/hbm_probe, built by `make -C vonkarman_amd/csrc probe`):
-    contiguous copies with one and several loads in flight per thread
-    and tiles of T adjacent 16-byte columns at 384 positions, the pattern of
-    the strided x and y passes (T = 8 there, 128-byte rows)."""
+    contiguous copies with one and several loads in flight per thread, tiles
+    of T adjacent 16-byte columns at 384 positions (the pattern of the
+    strided x and y passes, T = 8 there, 128-byte rows), the z pass's loads
+    and stores without its transforms (output lines written in part or
+    whole), and a copy small enough to stay in the last-level cache."""
     import subprocess
 
     exe = os.path.join(ROOT, "vonkarman_amd", "lib", "hbm_probe")
