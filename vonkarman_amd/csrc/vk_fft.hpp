@@ -531,9 +531,9 @@ __device__ __forceinline__ void fft_reg(cpx<R> (&v)[E], cpx<R> *lds, const cpx<R
     fft_reg_list<L, D, T, NT, SP, SC>(PL{}, v, lds, opaque(tw), tid);
 }
 
-// The same transform with the twiddle table in LDS (ltw, L entries, filled
-// by stage_twiddles).  No vector memory instruction runs inside the
-// transform, so HBM loads issued before it (a prefetch of the next field)
+// The same transform with the twiddle table in LDS (ltw, the first
+// tw_entries<L, PL>() entries of W_L, filled by twiddles_load and
+// twiddles_store).  No vector memory instruction runs inside the transform, so HBM loads issued before it (a prefetch of the next field)
 // stay in flight until their data is used: the memory counter that a twiddle
 // load would wait on counts those loads too, in issue order.
 template <int L, int D, int T, int NT, int SP, int SC, typename PL, int E, typename R>
@@ -542,28 +542,52 @@ __device__ __forceinline__ void fft_reg_lt(cpx<R> (&v)[E], cpx<R> *lds, const cp
     fft_reg_list<L, D, T, NT, SP, SC>(PL{}, v, lds, ltw, tid);
 }
 
-// copy the twiddle table of length L into LDS; the caller synchronises
+// Twiddle-table entries a plan reads: a pass of radix R run after NS
+// positions are done reads w = W_L^{(j % NS) L/(NS R)}, an index below L/R,
+// and the first pass reads none.  Covers the plan and its reverse (kernels
+// that chain an inverse and a forward transform share one LDS table).
+template <typename PL>
+constexpr int tw_reads(int L) {
+    int m = 0;
+    for (int i = 1; i < PL::n; ++i)
+        if (m == 0 || PL::r[i] < m) m = PL::r[i];
+    return m == 0 ? 1 : L / m;
+}
+template <int L, typename PL>
+constexpr int tw_entries() {
+    const int a = tw_reads<PL>(L), b = tw_reads<typename Reverse<PL>::type>(L);
+    return a > b ? a : b;
+}
+
+// Copy of the first W twiddles into LDS in two steps, so the caller can issue
+// its first HBM loads between them: twiddles_load issues the table loads,
+// twiddles_store writes LDS (waiting only for those loads, which were issued
+// first).  Otherwise every block would wait one memory round trip for the
+// table before its first field loads even start.  The caller synchronises
 // (the first transform's barrier comes after its first pass, which reads no
-// twiddles, so a __syncthreads() before the first transform is enough)
-// All loads are issued before the first LDS write (a load feeding a write in
-// the same loop trip is waited for before the next trip's load is issued),
-// and no write is predicated (the compiler would sink its load into the
-// branch): threads past the end load and write entry L-1, the value its own
-// thread writes.
-template <int L, int NT, typename R>
-__device__ __forceinline__ void stage_twiddles(cpx<R> *ltw, const cpx<R> *__restrict__ tw, int tid) {
-    const gptr<cpx<R>> g = (gptr<cpx<R>>)tw;
-    constexpr int C = (L + NT - 1) / NT;
+// twiddles).  No write is predicated (the compiler would sink its load into
+// the branch): threads past the end load and write entry W-1, the value its
+// own thread writes.
+template <int W, int NT, typename R>
+struct TwStage {
+    static constexpr int C = (W + NT - 1) / NT;
     cpx<R> t[C];
+};
+template <int W, int NT, typename R>
+__device__ __forceinline__ void twiddles_load(TwStage<W, NT, R> &s, const cpx<R> *__restrict__ tw, int tid) {
+    const gptr<cpx<R>> g = (gptr<cpx<R>>)tw;
 #pragma unroll
-    for (int c = 0; c < C; ++c) {
+    for (int c = 0; c < TwStage<W, NT, R>::C; ++c) {
         const int i = tid + c * NT;
-        t[c] = ldg(g, i < L ? i : L - 1);
+        s.t[c] = ldg(g, i < W ? i : W - 1);
     }
+}
+template <int W, int NT, typename R>
+__device__ __forceinline__ void twiddles_store(cpx<R> *ltw, const TwStage<W, NT, R> &s, int tid) {
 #pragma unroll
-    for (int c = 0; c < C; ++c) {
+    for (int c = 0; c < TwStage<W, NT, R>::C; ++c) {
         const int i = tid + c * NT;
-        ltw[i < L ? i : L - 1] = t[c];
+        ltw[i < W ? i : W - 1] = s.t[c];
     }
 }
 #endif

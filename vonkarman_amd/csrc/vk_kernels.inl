@@ -137,13 +137,19 @@ __global__ __launch_bounds__((InvGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
     // the 12-element plan, and the FP32 x pass: a 24-element FP64 prefetch
     // buffer would cost occupancy)
     constexpr bool PF = MODE != INV_PLAIN && (E <= 12 || (sizeof(TW) == 4 && (MODE == INV_U3DX || MODE == INV_UOMEGA)));
+    constexpr int TWN = PF ? tw_entries<L, PL>() : 1;
     __shared__ cpx<TW> lds[L * T];
-    __shared__ cpx<TW> ltw[PF ? L : 1];
+    __shared__ cpx<TW> ltw[TWN];
     const int tid = threadIdx.x;
     const int col0 = blockIdx.x * T;
     const cpx<TW> zero = {TW(0), TW(0)};
     cpx<TW> v[E];
-    if constexpr (PF) stage_twiddles<L, NT>(ltw, tw, tid);
+    TwStage<TWN, NT, TW> tws;
+    if constexpr (PF) twiddles_load(tws, tw, tid);
+    // the LDS twiddle writes follow the first field loads (see twiddles_load)
+    auto commit = [&] {
+        if constexpr (PF) twiddles_store(ltw, tws, tid);
+    };
 
     // this thread's column (NT is a multiple of T) and its base offsets
     const int col = col0 + tid % T;
@@ -234,6 +240,7 @@ __global__ __launch_bounds__((InvGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
     } else if constexpr (MODE == INV_U3) {
         fill(A, 0);
         fill(Q, 1);
+        commit();
         emit(0, copyA);
         take(A);
         fill(Q, 2);
@@ -245,6 +252,7 @@ __global__ __launch_bounds__((InvGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
         // outputs: 0 ux, 1 uy, 2 uz, 3 i kx uz, 4 i kx uy
         fill(A, 0);
         fill(Q, 1);
+        commit();
         emit(0, copyA);
         take(A);
         fill(Q, 2);
@@ -260,6 +268,7 @@ __global__ __launch_bounds__((InvGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
         constexpr int OW = (MODE == INV_UOMEGA) ? 3 : 0;
         fill(A, 0);
         fill(Q, 3);
+        commit();
         if constexpr (MODE == INV_UOMEGA) emit(0, copyA);
         take(B);
         fill(Q, 4);
@@ -631,12 +640,14 @@ __global__ __launch_bounds__((FwdGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
     // (twiddles from LDS, so no vector memory wait inside the transform holds
     // for the prefetch)
     constexpr bool PF = MODE == FWD_STAGE;
+    constexpr int TWN = PF ? tw_entries<L, PL>() : 1;
     __shared__ cpx<TW> lds[L * T];
-    __shared__ cpx<TW> ltw[PF ? L : 1];
+    __shared__ cpx<TW> ltw[TWN];
     const int tid = threadIdx.x;
     const int col0 = blockIdx.x * T;
     cpx<TW> v[E];
-    if constexpr (PF) stage_twiddles<L, NT>(ltw, tw, tid);
+    TwStage<TWN, NT, TW> tws;
+    if constexpr (PF) twiddles_load(tws, tw, tid);
     // kept outputs of the last radix under truncation: m in [0, RP/3) u [2RP/3, RP)
     constexpr bool KEEP = MODE == FWD_STAGE || MODE == FWD_VJP;
     constexpr int KR = KEEP ? 2 * RP / 3 : 1;
@@ -761,6 +772,8 @@ __global__ __launch_bounds__((FwdGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
 #pragma unroll
         for (int f = 0; f < 3; ++f) {
             if (f + 1 < 3) load_into(Q, f + 1);
+            // the LDS twiddle writes follow the first field loads (see twiddles_load)
+            if (f == 0) twiddles_store(ltw, tws, tid);
             fft_reg_lt<L, -1, T, NT, T, 1, PL>(v, lds, ltw, tid);
 #pragma unroll
             for (int b = 0; b < E / RP; ++b)
@@ -877,13 +890,15 @@ __global__ __launch_bounds__((FusedGeom<L, STG, TW>::NT)) __attribute__((amdgpu_
     // lane in stages 1-2 (fused pass 0.663 -> 0.648 ms with one, same box).
     constexpr int CH = sizeof(TW) == 8 || HEAVY ? 1 : 4;
     static_assert(K % CH == 0, "chunks must tile the kept modes");
+    constexpr int TWN = tw_entries<L, PL>();
     __shared__ cpx<TW> lds[L * T];
-    __shared__ cpx<TW> ltw[L];
+    __shared__ cpx<TW> ltw[TWN];
     const int tid = threadIdx.x;
     const int col0 = blockIdx.x * T;
     cpx<TW> v[E], Q[E];
     cpx<TW> r[3][K];
-    stage_twiddles<L, NT>(ltw, tw, tid);
+    TwStage<TWN, NT, TW> tws;
+    twiddles_load(tws, tw, tid);
 
     const int col = col0 + tid % T;
     const bool colok = col < g.ncols && (col % g.hp) < g.hvalid;
@@ -904,6 +919,8 @@ __global__ __launch_bounds__((FusedGeom<L, STG, TW>::NT)) __attribute__((amdgpu_
 #pragma unroll
     for (int f = 0; f < 3; ++f) {
         if (f + 1 < 3) load_into(Q, f + 1);
+        // the LDS twiddle writes follow the first field loads (see twiddles_load)
+        if (f == 0) twiddles_store(ltw, tws, tid);
         fft_reg_lt<L, -1, T, NT, T, 1, PL>(v, lds, ltw, tid);
 #pragma unroll
         for (int b = 0; b < E / RP; ++b)
@@ -1144,11 +1161,13 @@ __global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) __attribute__((amdgpu_waves_p
     constexpr int E = P::E;
     constexpr int T = ZNLGeom<L, TW>::T, NT = ZNLGeom<L, TW>::NT;
     constexpr int HIN = L / 3 + 1;  // N/2+1 with N = 2L/3
+    constexpr int TWN = tw_entries<L, PL>();
     __shared__ cpx<TW> lds[L * T];
-    __shared__ cpx<TW> ltw[L];
+    __shared__ cpx<TW> ltw[TWN];
     const int tid = threadIdx.x;
     const long long line0 = (long long)blockIdx.x * T;
-    stage_twiddles<L, NT>(ltw, tw, tid);
+    TwStage<TWN, NT, TW> tws;
+    twiddles_load(tws, tw, tid);
     const int t = tid % T;
     const bool ok = line0 + t < lines;
     // lines past the end load the last line (valid memory); their results are not stored
@@ -1166,6 +1185,8 @@ __global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) __attribute__((amdgpu_waves_p
                                                       opaque(in) + (2 * q + 1) * fs_in, base, tid);
     };
     issue(0);
+    // the LDS twiddle writes follow the first pair's loads (see twiddles_load)
+    twiddles_store(ltw, tws, tid);
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
         z_pair_combine<L, HIN, PL::first, T, NT, true>(v, raw, tid);
@@ -1268,11 +1289,13 @@ __global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) __attribute__((amdgpu_waves_p
     constexpr int E = P::E;
     constexpr int T = ZNLGeom<L, TW>::T, NT = ZNLGeom<L, TW>::NT;
     constexpr int HIN = L / 3 + 1, HOUT = HIN;
+    constexpr int TWN = tw_entries<L, PL>();
     __shared__ cpx<TW> lds[L * T];
-    __shared__ cpx<TW> ltw[L];
+    __shared__ cpx<TW> ltw[TWN];
     const int tid = threadIdx.x;
     const long long line0 = (long long)blockIdx.x * T;
-    stage_twiddles<L, NT>(ltw, tw, tid);
+    TwStage<TWN, NT, TW> tws;
+    twiddles_load(tws, tw, tid);
     const int t = tid % T;
     const bool ok = line0 + t < lines;
     // lines past the end load the last line (valid memory); their results are not stored
@@ -1284,10 +1307,15 @@ __global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) __attribute__((amdgpu_waves_p
     // inverse pair transform of fields (fa, fb) (fb < 0: none) with per-half
     // DC factors: the z transpose of an R2C counts its k=0 term once where the
     // C2R extension would count it twice
-    auto inverse = [&](int fa, int fb, TW dca, TW dcb) {
+    // (first: the LDS twiddle writes follow this transform's loads, see twiddles_load)
+    auto inverse = [&](int fa, int fb, TW dca, TW dcb, bool first = false) {
         const gptr<cpx<TW>> pa = opaque(in) + fa * fs_in;
-        if (fb >= 0) z_pair_fill<L, HIN, PL::first, T, NT, true>(v, pa, opaque(in) + fb * fs_in, base, tid);
-        else z_pair_fill<L, HIN, PL::first, T, NT, false>(v, pa, pa, base, tid);
+        ZRaw<TW> raw[E];
+        if (fb >= 0) z_pair_issue<L, HIN, PL::first, T, NT, true>(raw, pa, opaque(in) + fb * fs_in, base, tid);
+        else z_pair_issue<L, HIN, PL::first, T, NT, false>(raw, pa, pa, base, tid);
+        if (first) twiddles_store(ltw, tws, tid);
+        if (fb >= 0) z_pair_combine<L, HIN, PL::first, T, NT, true>(v, raw, tid);
+        else z_pair_combine<L, HIN, PL::first, T, NT, false>(v, raw, tid);
         // position 0 is element 0 of thread column 0's first row
         if (ci_pos_in<L, PL::first, T, NT>(tid, 0) == 0) v[0] = {v[0].re * dca, v[0].im * dcb};
         zfft<L, +1, T, NT, PL>(v, lds, ltw, tid);
@@ -1296,7 +1324,7 @@ __global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) __attribute__((amdgpu_waves_p
     // of (a + i b) leaks roundoff of the larger of a, b into the other, and u, w
     // are many orders above g.
     // g = R2C^T gin: C2R with the DC counted once, times 1/2
-    inverse(6, 7, two, two);
+    inverse(6, 7, two, two, true);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         g0[e] = v[e].re * half;
@@ -1372,11 +1400,13 @@ __global__ __launch_bounds__((ZMaxGeom<L, TW>::NT)) void k_z_max(const cpx<TW> *
     constexpr int E = P::E;
     constexpr int T = ZMaxGeom<L, TW>::T, NT = ZMaxGeom<L, TW>::NT;
     constexpr int HIN = L / 2 + 1;
+    constexpr int TWN = tw_entries<L, PL>();
     __shared__ cpx<TW> lds[L * T];
-    __shared__ cpx<TW> ltw[L];
+    __shared__ cpx<TW> ltw[TWN];
     __shared__ double wmax[NT / 64];
     const int tid = threadIdx.x;
-    stage_twiddles<L, NT>(ltw, tw, tid);
+    TwStage<TWN, NT, TW> tws;
+    twiddles_load(tws, tw, tid);
     const long long line0 = (long long)blockIdx.x * T;
     const int t = tid % T;
     const bool ok = line0 + t < lines;
@@ -1388,6 +1418,8 @@ __global__ __launch_bounds__((ZMaxGeom<L, TW>::NT)) void k_z_max(const cpx<TW> *
     ZRaw<TW> raw[E];
     const gptr<cpx<TW>> f2 = opaque(in) + 2 * fs;
     z_pair_issue<L, HIN, PL::first, T, NT, true>(raw, opaque(in), opaque(in) + fs, base, tid);
+    // the LDS twiddle writes follow the first pair's loads (see twiddles_load)
+    twiddles_store(ltw, tws, tid);
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
         if (q == 0) {
@@ -1435,10 +1467,12 @@ __global__ __launch_bounds__((ZSGeom<L, TW>::NT)) void k_z_c2r(const cpx<TW> *__
     constexpr int E = P::E;
     constexpr int T = ZSGeom<L, TW>::T, NT = ZSGeom<L, TW>::NT;
     constexpr int HIN = L / 2 + 1;
+    constexpr int TWN = tw_entries<L, PL>();
     __shared__ cpx<TW> lds[L * T];
-    __shared__ cpx<TW> ltw[L];
+    __shared__ cpx<TW> ltw[TWN];
     const int tid = threadIdx.x;
-    stage_twiddles<L, NT>(ltw, tw, tid);
+    TwStage<TWN, NT, TW> tws;
+    twiddles_load(tws, tw, tid);
     const long long s = (long long)blockIdx.x * T + tid % T;
     const long long la = 2 * s, lb = la + 1;
     cpx<TW> v[E];
@@ -1448,6 +1482,8 @@ __global__ __launch_bounds__((ZSGeom<L, TW>::NT)) void k_z_c2r(const cpx<TW> *__
         ZRaw<TW> ra[E], rb[E];
         z_pair_issue<L, HIN, PL::first, T, NT, false>(ra, src, src, (la < lines ? la : lines - 1) * hp, tid);
         z_pair_issue<L, HIN, PL::first, T, NT, false>(rb, src, src, (lb < lines ? lb : lines - 1) * hp, tid);
+        // the LDS twiddle writes follow the lines' loads (see twiddles_load)
+        twiddles_store(ltw, tws, tid);
         cpx<TW> vb[E];
         z_pair_combine<L, HIN, PL::first, T, NT, false>(v, ra, tid);
         z_pair_combine<L, HIN, PL::first, T, NT, false>(vb, rb, tid);
@@ -1475,10 +1511,12 @@ __global__ __launch_bounds__((ZSGeom<L, TW>::NT)) void k_z_r2c(const double *__r
     constexpr int E = P::E;
     constexpr int T = ZSGeom<L, TW>::T, NT = ZSGeom<L, TW>::NT;
     constexpr int HOUT = L / 2 + 1;
+    constexpr int TWN = tw_entries<L, PL>();
     __shared__ cpx<TW> lds[L * T];
-    __shared__ cpx<TW> ltw[L];
+    __shared__ cpx<TW> ltw[TWN];
     const int tid = threadIdx.x;
-    stage_twiddles<L, NT>(ltw, tw, tid);
+    TwStage<TWN, NT, TW> tws;
+    twiddles_load(tws, tw, tid);
     const long long pair0 = (long long)blockIdx.x * T;
     const int t = tid % T;
     const long long la = 2 * (pair0 + t), lb = la + 1;
@@ -1492,6 +1530,8 @@ __global__ __launch_bounds__((ZSGeom<L, TW>::NT)) void k_z_r2c(const double *__r
         const double a = src[ca + p], b = src[cb + p];
         v[e] = {la < lines ? TW(a) : TW(0), lb < lines ? TW(b) : TW(0)};
     }
+    // the LDS twiddle writes follow the lines' loads (see twiddles_load)
+    twiddles_store(ltw, tws, tid);
     fft_reg_lt<L, -1, T, NT, T, 1, PL>(v, lds, ltw, tid);
     __syncthreads();
 #pragma unroll
