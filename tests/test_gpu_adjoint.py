@@ -115,12 +115,17 @@ def test_checkpoint_schedules_give_identical_gradients(hip_device):
     u = hermitian_field(n, 7)
     s = _gpu_solver(n, nu, dt, u)
     res = [s.adjoint(T, "final_enstrophy", max_snapshots=k) for k in (1, 2, 4, T)]
-    for J, g, _ in res[1:]:
+    # recomputing schedules agree bit for bit; with every state stored the
+    # forward sweep also stores the stage inputs (no recompute), which agrees
+    # to rounding (the fused and standalone x passes round differently)
+    for J, g, _ in res[1:3]:
         assert J == res[0][0]
         assert np.array_equal(g, res[0][1])
+    assert res[3][0] == res[0][0]
+    assert rel(res[3][1], res[0][1]) < 1e-13
     # fewer snapshots, more recomputation; all T states visited
     steps = [fs for _, _, fs in res]
-    assert steps[0] >= steps[1] >= steps[2] >= steps[3] >= T
+    assert steps[0] >= steps[1] >= steps[2] >= steps[3] == T
     assert np.array_equal(s.u_hat(), u) and s.step_count() == 0
 
 

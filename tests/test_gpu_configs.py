@@ -105,7 +105,8 @@ def test_iso_256_mixed_adjoint_tracks_fp64(hip_device):
     """The mixed configuration's objective and grid: J = sum_n dt 2 nu
     Omega(u_n) over 10 steps at dt = 5e-4 on the von Karman field at 256^3.
     The mixed gradient agrees with the FP64 one to 1e-5 (rel. L2) and J to
-    1e-6, and is bitwise the same with 3 or 10 stored states."""
+    1e-6.  With 10 stored states (the stage inputs stored as well, no
+    recompute) it agrees with the 3-snapshot schedule to FP32 rounding."""
     from vonkarman_amd import ic
 
     n, nu, dt, T = 256, 1e-3, 5e-4, 10
@@ -119,5 +120,6 @@ def test_iso_256_mixed_adjoint_tracks_fp64(hip_device):
     Jmx10, gmx10, fs10 = smx.adjoint(T, "dissipation", max_snapshots=10)
     assert abs(Jmx - J64) < 1e-6 * J64
     assert rel(gmx, g64) < 1e-5
-    assert Jmx10 == Jmx and np.array_equal(gmx10, gmx)
+    print(f"stored stages vs recompute: rel {rel(gmx10, gmx):.3e}", flush=True)
+    assert Jmx10 == Jmx and rel(gmx10, gmx) < 1e-6
     assert fs3 > fs10 == T

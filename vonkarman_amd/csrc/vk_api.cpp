@@ -356,6 +356,9 @@ struct vk_solver {
     DevBuf KB[3];
     bool keep_b = false;
     std::vector<DevBuf> snaps;                 // checkpointed states
+    // stage inputs a, b, c of every step (3 per step), stored by the forward
+    // sweep of a gradient whose states all fit (see adjoint_run)
+    std::vector<DevBuf> stage_snaps;
     std::vector<int> split_tab;                // binomial schedule: best first advance per (steps, free slots)
     int split_T = 0, split_S = 0;
     Scalars *dsc() { return static_cast<Scalars *>(sc.p); }
@@ -488,7 +491,9 @@ int cfl_async(vk_solver *s) {
     return 0;
 }
 
-int one_step(vk_solver *s, vk_timing *t) {
+// stage_out: when set, the stage inputs a, b, c of this step are also stored
+// there (three state-layout arrays)
+int one_step(vk_solver *s, vk_timing *t, void *const *stage_out = nullptr) {
     float ms = 0;
     // untimed adaptive steps overlap the CFL pass with the first stage's
     // passes before its epilogue; timed steps run it alone (its own time)
@@ -516,8 +521,8 @@ int one_step(vk_solver *s, vk_timing *t) {
         // (restarts, adjoint snapshots): the fused and standalone x passes
         // agree only to rounding.
         const bool skip = s->fuse_x && k > 0, fuse = s->fuse_x && k < 3;
-        const void *src = k == 0 ? s->u.p : s->x.p;
-        void *dst = k == 3 ? s->u.p : (fuse ? nullptr : s->x.p);
+        const void *src = k == 0 ? s->u.p : (stage_out ? stage_out[k - 1] : s->x.p);
+        void *dst = k == 3 ? s->u.p : (stage_out ? stage_out[k] : (fuse ? nullptr : s->x.p));
         StageArgs sa = stage_args(s, stages[k], dst);
         if (int r = nl_eval(s, src, sa, t, s->ev, skip, fuse, nullptr, overlap && k == 0 ? s->ev_cfl : nullptr))
             return r;
@@ -617,6 +622,7 @@ int vk_solver_destroy(vk_solver *s) {
                       &s->cot, &s->WA, &s->WB, &s->C1, &s->C2})
         dev_free(*b);
     for (DevBuf &b : s->snaps) dev_free(b);
+    for (DevBuf &b : s->stage_snaps) dev_free(b);
     for (DevBuf &b : s->KB) dev_free(b);
     for (auto &e : s->ev)
         if (e) (void)hipEventDestroy(e);
@@ -965,6 +971,25 @@ int vjp_nl(vk_solver *s, const void *v, StageArgs &sa, void *pre = nullptr) {
     return 0;
 }
 
+// the reverse sweep of one step at the state s->u with stage inputs a, b, c:
+// s->lam <- (d step / d u)^T s->lam.  kb: the kept y-pass outputs of u, a, b
+// (nullptr: the VJP runs their x and y passes)
+int step_vjp_reverse(vk_solver *s, const void *a, const void *b, const void *c, void *const kb[3]) {
+    StageArgs tmp = adj_args(s, AS_4, nullptr);
+    VK_TRY(launch_adj_prep(static_cast<const double *>(s->lam.p), static_cast<double *>(s->cot.p), s->n, s->hp,
+                           s->S, s->ks, tmp.inv_scale, static_cast<const double *>(s->tab.p), s->qmax + 1,
+                           s->dsc(), 1, s->st));
+    StageArgs a4 = adj_args(s, AS_4, nullptr);
+    if (int r = vjp_nl(s, c, a4)) return r;
+    StageArgs a3 = adj_args(s, AS_3, nullptr);
+    if (int r = vjp_nl(s, b, a3, kb[2])) return r;
+    StageArgs a2 = adj_args(s, AS_2, nullptr);
+    if (int r = vjp_nl(s, a, a2, kb[1])) return r;
+    StageArgs a1 = adj_args(s, AS_1, s->lam.p);
+    if (int r = vjp_nl(s, s->u.p, a1, kb[0])) return r;
+    return 0;
+}
+
 // s->lam <- (d step / d u)^T s->lam at the state s->u (fixed dt); u unchanged
 int step_vjp(vk_solver *s) {
     if (int r = ensure_vjp(s)) return r;
@@ -981,20 +1006,16 @@ int step_vjp(vk_solver *s) {
     if (int r = nl_eval(s, s->sa_.p, f2, nullptr, nullptr, fz, fz, kb[1])) return r;
     StageArgs f3 = stage_args(s, ST_3, s->sc_.p);
     if (int r = nl_eval(s, s->sb_.p, f3, nullptr, nullptr, fz, false, kb[2])) return r;
-    // reverse sweep
-    StageArgs tmp = adj_args(s, AS_4, nullptr);
-    VK_TRY(launch_adj_prep(static_cast<const double *>(s->lam.p), static_cast<double *>(s->cot.p), s->n, s->hp,
-                           s->S, s->ks, tmp.inv_scale, static_cast<const double *>(s->tab.p), s->qmax + 1,
-                           s->dsc(), 1, s->st));
-    StageArgs a4 = adj_args(s, AS_4, nullptr);
-    if (int r = vjp_nl(s, s->sc_.p, a4)) return r;
-    StageArgs a3 = adj_args(s, AS_3, nullptr);
-    if (int r = vjp_nl(s, s->sb_.p, a3, kb[2])) return r;
-    StageArgs a2 = adj_args(s, AS_2, nullptr);
-    if (int r = vjp_nl(s, s->sa_.p, a2, kb[1])) return r;
-    StageArgs a1 = adj_args(s, AS_1, s->lam.p);
-    if (int r = vjp_nl(s, s->u.p, a1, kb[0])) return r;
-    return 0;
+    return step_vjp_reverse(s, s->sa_.p, s->sb_.p, s->sc_.p, kb);
+}
+
+// s->lam <- (d step / d u)^T s->lam at the state s->u, whose stage inputs a,
+// b, c the forward step stored (no recompute; the four VJPs run their own
+// state x and y passes)
+int step_vjp_stored(vk_solver *s, const void *a, const void *b, const void *c) {
+    if (int r = ensure_vjp(s)) return r;
+    void *const kb[3] = {nullptr, nullptr, nullptr};
+    return step_vjp_reverse(s, a, b, c, kb);
 }
 
 // unpadded complex double (host or device) -> a state-layout array of precision dp
@@ -1041,6 +1062,9 @@ struct Adjoint {
     vk_solver *s;
     int T, S, objective;
     bool lamT_given;
+    // every step's stage inputs are stored by its forward step (all states
+    // fit: every advance in rev is one step, every leaf one step)
+    bool store = false;
     double J = 0.0;
     long long forward_steps = 0;
     double nt2;
@@ -1054,6 +1078,19 @@ struct Adjoint {
             ++forward_steps;
         }
         return 0;
+    }
+    void *const *stages_of(int step) { return stage_ptr.data() + 3 * (size_t)step; }
+    std::vector<void *> stage_ptr;
+    // one step from s->u (the state at `step`), storing its stage inputs
+    int advance_storing(int step) {
+        if (int r = one_step(s, nullptr, stages_of(step))) return r;
+        ++forward_steps;
+        return 0;
+    }
+    int reverse_step(int step) {
+        if (!store) return step_vjp(s);
+        void *const *p = stages_of(step);
+        return step_vjp_stored(s, p[0], p[1], p[2]);
     }
     double enstrophy() {
         if (launch_diag(s->u.p, s->wp, s->n, s->hp, s->S, s->ks, s->nshells,
@@ -1098,18 +1135,18 @@ struct Adjoint {
                 if (at == T - 1) {  // the step into u_T: form u_T, its objective terms, then back
                     // sa_ is rewritten by every step VJP, so it is free here
                     if (int r = copy_state(s->sa_.p, s->u.p)) return r;
-                    if (int r = advance(1)) return r;
+                    if (int r = store ? advance_storing(at) : advance(1)) return r;
                     if (int r = objective_at(T, true)) return r;
                     if (int r = copy_state(s->u.p, s->sa_.p)) return r;
                 }
-                if (int r = step_vjp(s)) return r;
+                if (int r = reverse_step(at)) return r;
                 if (int r = objective_at(at, false)) return r;
             }
             return 0;
         }
         const int m = s->split_tab[(size_t)n * s->split_S + nfree];
         if (int r = copy_state(s->u.p, s->snaps[slot].p)) return r;
-        if (int r = advance(m)) return r;
+        if (int r = store && m == 1 ? advance_storing(lo) : advance(m)) return r;
         if (int r = copy_state(s->snaps[first].p, s->u.p)) return r;
         if (int r = rev(lo + m, n - m, first, first + 1, nfree - 1)) return r;
         return rev(lo, m, slot, first, nfree);
@@ -1132,6 +1169,34 @@ int adjoint_run(vk_solver *s, int nsteps, int objective, const double *lamT, int
         s->snaps.push_back(b);
     }
     build_schedule(s, nsteps, S);
+    // When every state fits, the forward sweep also stores each step's stage
+    // inputs (3 states per step) if they fit with 8 GB to spare: the reverse
+    // steps then skip the recompute of a, b, c (three nonlinear evaluations).
+    // The gradient then agrees with a recomputing schedule's to rounding, not
+    // bit for bit: the step computes c in the fused stage pass (the recompute
+    // in the unfused one), and the VJPs at a and b run the standalone x pass
+    // where the recompute keeps the fused pass's x inverse; the two round
+    // differently in the last bit.  Recomputing schedules agree bit for bit
+    // among themselves.
+    bool store = false;
+    if (S >= nsteps) {
+        const size_t have = s->stage_snaps.size(), want = (size_t)3 * nsteps;
+        const size_t need = want > have ? (want - have) * s->state_bytes() : 0;
+        size_t free_b = 0, total_b = 0;
+        if (need == 0 || (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b > need + (size_t(8) << 30))) {
+            store = true;
+            while (s->stage_snaps.size() < want) {
+                DevBuf b;
+                if (dev_alloc(b, s->state_bytes(), s->total)) {
+                    store = false;  // allocation failed: recompute instead
+                    g_err.clear();
+                    (void)hipGetLastError();  // the failed hipMalloc's error is not this call's
+                    break;
+                }
+                s->stage_snaps.push_back(b);
+            }
+        }
+    }
     Scalars h0;
     if (int r = read_scalars(s, h0)) return r;
     VK_HIP(hipMemsetAsync(s->lam.p, 0, (size_t)s->S * 3 * 16, s->st));
@@ -1143,6 +1208,9 @@ int adjoint_run(vk_solver *s, int nsteps, int objective, const double *lamT, int
     A.S = S;
     A.objective = objective;
     A.lamT_given = lamT != nullptr;
+    A.store = store;
+    if (store)
+        for (int i = 0; i < 3 * nsteps; ++i) A.stage_ptr.push_back(s->stage_snaps[i].p);
     const double nt = (double)s->n * s->n * s->n;
     A.nt2 = nt * nt;
     if (int r = A.copy_state(s->snaps[0].p, s->u.p)) return r;
