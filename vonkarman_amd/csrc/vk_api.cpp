@@ -962,12 +962,13 @@ int vjp_nl(vk_solver *s, const void *v, StageArgs &sa, void *pre = nullptr) {
     // z: GU = C2R^T(w x g), GW = C2R^T(g x u) -> WA (6 fields)
     VK_TRY(launch_z_vjp(M, s->wp, wb9, s->WA.p, (long long)M * M, Hp, MMH, MMH, 1.0 / nt, 1.0 / nt, s->tw_m.p,
                         s->st));
-    // y forward + truncation of the 6 fields -> WB (y-major)
+    // y forward + truncation of the 6 fields with the ky and kz curl terms of
+    // t = GU + curl GW formed -> 5 fields in WB (y-major)
     Strided g3 = make_strided((int)MH, Hp, H, Hp, MH, Hp, MMH, (int)MH, 0, MH, NMH);
-    VK_TRY(launch_fwd(FWD_PLAIN, M, N, 6, s->wp, s->WA.p, s->WB.p, s->tw_m.p, g3, N, s->ks, 1.0, nullptr, s->st));
-    // x forward + truncation, t = GU + curl GW, reverse-stage epilogue
+    VK_TRY(launch_fwd(FWD_VJPY, M, N, 6, s->wp, s->WA.p, s->WB.p, s->tw_m.p, g3, N, s->ks, 1.0, nullptr, s->st));
+    // x forward + truncation, the kx curl terms, reverse-stage epilogue
     Strided g4 = make_strided((int)NH, Hp, H, Hp, MH, Hp, NMH, (int)NH, 0, NH, s->S);
-    VK_TRY(launch_fwd(FWD_VJP, M, N, 6, s->wp, s->WB.p, nullptr, s->tw_m.p, g4, N, s->ks, 1.0, &sa, s->st));
+    VK_TRY(launch_fwd(FWD_VJP, M, N, 5, s->wp, s->WB.p, nullptr, s->tw_m.p, g4, N, s->ks, 1.0, &sa, s->st));
     return 0;
 }
 

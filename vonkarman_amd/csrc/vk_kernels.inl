@@ -288,8 +288,9 @@ __global__ __launch_bounds__((InvGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
 // ===========================================================================
 // Forward strided pass: lines of L physical values, forward DFT, the LOUT kept
 // modes (3/2-rule truncation map) scaled and stored.  FWD_STAGE additionally
-// projects (Leray) and runs one ETD-RK4 stage epilogue per mode; FWD_VJP forms
-// t = GU + curl GW and runs one reverse-stage epilogue per mode.  The stage is
+// projects (Leray) and runs one ETD-RK4 stage epilogue per mode; FWD_VJPY
+// (y pass) and FWD_VJP (x pass) form t = GU + curl GW between them, and
+// FWD_VJP runs one reverse-stage epilogue per mode.  The stage is
 // a template parameter, so each epilogue's loads are known at compile time and
 // are issued together per chunk of CH kept modes (the state arrays may alias
 // as far as the compiler knows, so loads of the next chunk wait behind the
@@ -623,6 +624,11 @@ template <int L, typename TW>
 struct FwdPlan<L, FWD_VJP, TW> {
     using type = PlanE12<L>;
 };
+// the VJP y pass holds one field of kept outputs beside the transform
+template <int L, typename TW>
+struct FwdPlan<L, FWD_VJPY, TW> {
+    using type = PlanE12<L>;
+};
 template <int L, int MODE, typename TW>
 using FwdGeom = Geom<L, FwdPlan<L, MODE, TW>::type::E, TW, kRowBytes>;
 
@@ -649,7 +655,7 @@ __global__ __launch_bounds__((FwdGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
     TwStage<TWN, NT, TW> tws;
     if constexpr (PF) twiddles_load(tws, tw, tid);
     // kept outputs of the last radix under truncation: m in [0, RP/3) u [2RP/3, RP)
-    constexpr bool KEEP = MODE == FWD_STAGE || MODE == FWD_VJP;
+    constexpr bool KEEP = MODE == FWD_STAGE || MODE == FWD_VJP || MODE == FWD_VJPY;
     constexpr int KR = KEEP ? 2 * RP / 3 : 1;
     constexpr int K = KEEP ? (E / RP) * KR : 1;
     static_assert(!KEEP || (RP % 3 == 0 && 3 * LOUT == 2 * L), "stage pass needs a 3/2-padded plan");
@@ -707,46 +713,35 @@ __global__ __launch_bounds__((FwdGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
         idx = (ok ? col_addr(c, g.out_inner, g.out_os) : 0) + (long long)j * g.out_ls;
         q = ok ? wx * wx + wy * wy + iz * iz : 0;
     };
+    // kept output i of the transform in v (scaled)
+    auto kept_v = [&](int i) -> cpx<TW> {
+        const int b = i / KR, mk = i % KR;
+        return cscale(v[b * RP + (mk < RP / 3 ? mk : mk + RP / 3)], scale);
+    };
+    auto ik = [](TW k, cpx<TW> a) -> cpx<TW> { return {-k * a.im, k * a.re}; };
     if constexpr (MODE == FWD_VJP) {
-        // t = GU + curl(GW): GW (fields 3..5) first, curled in registers, then GU added
-#pragma unroll
-        for (int f = 0; f < 3; ++f) {
-            load_field(3 + f);
-            fft_reg<L, -1, T, NT, T, 1, PL>(v, lds, tw, tid);
-#pragma unroll
-            for (int b = 0; b < E / RP; ++b)
-#pragma unroll
-                for (int mk = 0; mk < KR; ++mk) {
-                    const int m = mk < RP / 3 ? mk : mk + RP / 3;
-                    r[f][b * KR + mk] = cscale(v[b * RP + m], scale);
-                }
-            __syncthreads();
-        }
-#pragma unroll
-        for (int i = 0; i < K; ++i) {
+        // t = GU + curl(GW) (SpectralOps::curl, vonkarman-core/src/spectral_ops.rs:93-104)
+        // from the y pass's five fields: 0 t_x (complete), 1 t_y' = GU_y +
+        // i kz GW_x, 2 t_z' = GU_z - i ky GW_x, 3 GW_y, 4 GW_z; the i kx terms
+        // are added here: t_y = t_y' - i kx GW_z, t_z = t_z' + i kx GW_y
+        auto kx_of = [&](int i) -> TW {
             long long idx;
             int q, wx, wy, iz;
             kept(i, idx, q, wx, wy, iz);
-            const TW kx = TW(double(wx) * ks), ky = TW(double(wy) * ks), kz = TW(double(iz) * ks);
-            const cpx<TW> w0 = r[0][i], w1 = r[1][i], w2 = r[2][i];
-            // SpectralOps::curl (vonkarman-core/src/spectral_ops.rs:93-104)
-            r[0][i] = {-(ky * w2.im - kz * w1.im), ky * w2.re - kz * w1.re};
-            r[1][i] = {-(kz * w0.im - kx * w2.im), kz * w0.re - kx * w2.re};
-            r[2][i] = {-(kx * w1.im - ky * w0.im), kx * w1.re - ky * w0.re};
-        }
-#pragma unroll
-        for (int f = 0; f < 3; ++f) {
+            return TW(double(wx) * ks);
+        };
+        auto field = [&](int f, auto take) {
             load_field(f);
             fft_reg<L, -1, T, NT, T, 1, PL>(v, lds, tw, tid);
 #pragma unroll
-            for (int b = 0; b < E / RP; ++b)
-#pragma unroll
-                for (int mk = 0; mk < KR; ++mk) {
-                    const int m = mk < RP / 3 ? mk : mk + RP / 3;
-                    r[f][b * KR + mk] = cadd(r[f][b * KR + mk], cscale(v[b * RP + m], scale));
-                }
+            for (int i = 0; i < K; ++i) take(i, kept_v(i));
             __syncthreads();
-        }
+        };
+        field(3, [&](int i, cpx<TW> x) { r[2][i] = ik(kx_of(i), x); });
+        field(4, [&](int i, cpx<TW> x) { r[1][i] = ik(-kx_of(i), x); });
+        field(1, [&](int i, cpx<TW> x) { r[1][i] = cadd(x, r[1][i]); });
+        field(2, [&](int i, cpx<TW> x) { r[2][i] = cadd(x, r[2][i]); });
+        field(0, [&](int i, cpx<TW> x) { r[0][i] = x; });
 #pragma unroll
         for (int c0 = 0; c0 < K; c0 += CH) {
             long long idx[CH];
@@ -765,6 +760,49 @@ __global__ __launch_bounds__((FwdGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
             }
             adj_chunk<STG, CH>(sa, colok, idx, q, kv, tt);
         }
+    } else if constexpr (MODE == FWD_VJPY) {
+        // y forward of the z VJP pass's output (GU: fields 0..2, GW: 3..5),
+        // truncated, into the five fields the x VJP pass reads.  The curl
+        // terms of t = GU + curl GW along ky and kz are formed here (ky is this
+        // pass's wavenumber, kz the column's), the kx terms after the x
+        // transform, so one field of traffic and one x transform are saved:
+        //   0 t_x  = GU_x + i (ky GW_z - kz GW_y)   (complete)
+        //   1 t_y' = GU_y + i kz GW_x
+        //   2 t_z' = GU_z - i ky GW_x
+        //   3 GW_y, 4 GW_z
+        // Padding columns are stored too (into row padding), as in the plain pass.
+        constexpr bool PADP = pad_stores<TW, T>();
+        const bool stp = PADP || colok;
+        const long long coutp = stp ? col_addr(col, g.out_inner, g.out_os) : 0;
+        const TW kz = TW(double(col % g.hp) * ks);
+        auto jof = [&](int i) -> int {
+            const int b = i / KR, mk = i % KR;
+            return unpad_index(ci_pos_out<L, RP, T, NT>(tid, b * RP + (mk < RP / 3 ? mk : mk + RP / 3)), LOUT, L);
+        };
+        auto ky_of = [&](int i) -> TW { return TW(double(wavenum(jof(i), n)) * ks); };
+        auto put = [&](int fo, int i, cpx<TW> x) {
+            if (stp) out[fo * g.out_fs + coutp + (long long)jof(i) * g.out_ls] = x;
+        };
+        cpx<TW> h[K];  // GW_x, then i (ky GW_z - kz GW_y)
+        auto field = [&](int f, auto take) {
+            load_field(f);
+            fft_reg<L, -1, T, NT, T, 1, PL>(v, lds, tw, tid);
+#pragma unroll
+            for (int i = 0; i < K; ++i) take(i, kept_v(i));
+            __syncthreads();
+        };
+        field(3, [&](int i, cpx<TW> x) { h[i] = x; });
+        field(1, [&](int i, cpx<TW> x) { put(1, i, cadd(x, ik(kz, h[i]))); });
+        field(2, [&](int i, cpx<TW> x) { put(2, i, cadd(x, ik(-ky_of(i), h[i]))); });
+        field(4, [&](int i, cpx<TW> x) {
+            put(3, i, x);
+            h[i] = ik(-kz, x);
+        });
+        field(5, [&](int i, cpx<TW> x) {
+            put(4, i, x);
+            h[i] = cadd(h[i], ik(ky_of(i), x));
+        });
+        field(0, [&](int i, cpx<TW> x) { put(0, i, cadd(x, h[i])); });
     } else if constexpr (MODE == FWD_STAGE) {
         // three fields; the kept outputs of each stay in registers (static indices)
         cpx<TW> Q[E];
@@ -1620,7 +1658,7 @@ template <int L, int LOUT, int MODE, int NF, int STG, typename TW>
 int run_fwd(const void *in, void *out, const void *tw, const Strided &g, int n, double ks, double scale,
             const StageArgs *sa, hipStream_t st) {
     using C = FwdGeom<L, MODE, TW>;
-    if (MODE == FWD_PLAIN && NF != 1 && pad_stores<TW, C::T>() && g.ncols % C::T != 0) return 2;
+    if ((MODE == FWD_PLAIN || MODE == FWD_VJPY) && NF != 1 && pad_stores<TW, C::T>() && g.ncols % C::T != 0) return 2;
     const unsigned nb = (unsigned)((g.ncols + C::T - 1) / C::T);
     StageArgs a{};
     if (sa) a = *sa;
@@ -1651,11 +1689,11 @@ template <int L, int LOUT, typename TW>
 int run_fwd_vjp(const void *in, void *out, const void *tw, const Strided &g, int n, double ks, double scale,
                 const StageArgs *sa, hipStream_t st) {
     switch (sa->stage) {
-        case AS_RAW: return run_fwd<L, LOUT, FWD_VJP, 6, AS_RAW, TW>(in, out, tw, g, n, ks, scale, sa, st);
-        case AS_1: return run_fwd<L, LOUT, FWD_VJP, 6, AS_1, TW>(in, out, tw, g, n, ks, scale, sa, st);
-        case AS_2: return run_fwd<L, LOUT, FWD_VJP, 6, AS_2, TW>(in, out, tw, g, n, ks, scale, sa, st);
-        case AS_3: return run_fwd<L, LOUT, FWD_VJP, 6, AS_3, TW>(in, out, tw, g, n, ks, scale, sa, st);
-        case AS_4: return run_fwd<L, LOUT, FWD_VJP, 6, AS_4, TW>(in, out, tw, g, n, ks, scale, sa, st);
+        case AS_RAW: return run_fwd<L, LOUT, FWD_VJP, 5, AS_RAW, TW>(in, out, tw, g, n, ks, scale, sa, st);
+        case AS_1: return run_fwd<L, LOUT, FWD_VJP, 5, AS_1, TW>(in, out, tw, g, n, ks, scale, sa, st);
+        case AS_2: return run_fwd<L, LOUT, FWD_VJP, 5, AS_2, TW>(in, out, tw, g, n, ks, scale, sa, st);
+        case AS_3: return run_fwd<L, LOUT, FWD_VJP, 5, AS_3, TW>(in, out, tw, g, n, ks, scale, sa, st);
+        case AS_4: return run_fwd<L, LOUT, FWD_VJP, 5, AS_4, TW>(in, out, tw, g, n, ks, scale, sa, st);
     }
     return 2;
 }
@@ -1683,7 +1721,11 @@ int launch_fwd_L(int mode, int lout, int nf, Prec wp, const void *in, void *out,
             if (wp == F64) return run_fwd<L, LOUT, FWD_PLAIN, 6, 0, double>(in, out, tw, g, n, ks, scale, sa, st);
             return run_fwd<L, LOUT, FWD_PLAIN, 6, 0, float>(in, out, tw, g, n, ks, scale, sa, st);
         }
-        if (mode == FWD_VJP && nf == 6 && sa) {
+        if (mode == FWD_VJPY && nf == 6) {
+            if (wp == F64) return run_fwd<L, LOUT, FWD_VJPY, 6, 0, double>(in, out, tw, g, n, ks, scale, sa, st);
+            return run_fwd<L, LOUT, FWD_VJPY, 6, 0, float>(in, out, tw, g, n, ks, scale, sa, st);
+        }
+        if (mode == FWD_VJP && nf == 5 && sa) {
             if (wp == F64) return run_fwd_vjp<L, LOUT, double>(in, out, tw, g, n, ks, scale, sa, st);
             return run_fwd_vjp<L, LOUT, float>(in, out, tw, g, n, ks, scale, sa, st);
         }

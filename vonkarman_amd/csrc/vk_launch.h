@@ -63,7 +63,8 @@ int launch_inv(int mode, int L, int lin, Prec in_prec, Prec work_prec, const voi
 enum FwdMode {
     FWD_PLAIN = 0,  // nf fields, each truncated to lout positions
     FWD_STAGE = 1,  // 3 fields, truncate, Leray, then an ETD-RK4 stage epilogue
-    FWD_VJP = 2,    // 6 fields (GU, GW), truncate, t = GU + curl GW, then a reverse-stage epilogue
+    FWD_VJP = 2,    // x pass: the 5 FWD_VJPY fields, truncate, t = GU + curl GW, then a reverse-stage epilogue
+    FWD_VJPY = 3,   // y pass: 6 fields (GU, GW) -> 5 (t_x, t_y', t_z', GW_y, GW_z), truncate; see k_fwd
 };
 enum Stage {
     ST_RAW = 0, ST_1 = 1, ST_2 = 2, ST_3 = 3, ST_4 = 4,
