@@ -1373,44 +1373,45 @@ __global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) __attribute__((amdgpu_waves_p
 #pragma unroll
     for (int e = 0; e < E; ++e) r0[e] = v[e].re * half;  // g2
     __syncthreads();
-    inverse(0, 1, one, one);  // (ux, uy): gW2 = g0 U1 - g1 U0 and the U0, U1 parts of gW0, gW1
+    // W before U, holding W1, W2 and then U0 rather than partial products:
+    // at most g (3 fields), two held fields and the transform pair are live
+    inverse(4, 5, one, one);  // (wy, wz), held
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-        const TW u0 = v[e].re * s_u, u1 = v[e].im * s_u, gz = r0[e];
-        r1[e] = -gz * u1;                 // gW0 = g1 U2 - g2 U1
-        r2[e] = gz * u0;                  // gW1 = g2 U0 - g0 U2
-        r3[e] = g0[e] * u1 - g1[e] * u0;  // gW2
+        r1[e] = v[e].re * s_u;  // W1
+        r2[e] = v[e].im * s_u;  // W2
     }
     __syncthreads();
-    inverse(2, 3, one, one);  // (uz, wx): gW0, gW1 complete; the W0 parts of gU1, gU2
+    inverse(3, 0, one, one);  // (wx, ux): gU complete; U0 held
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-        const TW u2 = v[e].re * s_u, w0 = v[e].im * s_u, gz = r0[e];
-        const TW gw0 = r1[e] + g1[e] * u2, gw1 = r2[e] - g0[e] * u2;
-        r1[e] = -w0 * gz;     // gU1 = w2 g0 - w0 g2
-        r2[e] = w0 * g1[e];   // gU2 = w0 g1 - w1 g0
-        v[e] = {gw0, gw1};
-    }
-    __syncthreads();
-    zfft<L, -1, T, NT, RL>(v, lds, ltw, tid);
-    __syncthreads();
-    z_split_store<L, T, NT, RL, E, HOUT>(v, lds, tid, line0, lines, hp, out + 3 * fs_out, out + 4 * fs_out, s_o);
-    // live: g0, g1, g2 = r0, gU1 part = r1, gU2 part = r2, gW2 = r3
-    inverse(4, 5, one, one);  // (wy, wz): gU complete
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-        const TW w1 = v[e].re * s_u, w2 = v[e].im * s_u, gz = r0[e];
-        const TW gu0 = w1 * gz - w2 * g1[e];  // gU0 = w1 g2 - w2 g1
-        const TW gu1 = r1[e] + w2 * g0[e];
-        r0[e] = r2[e] - w1 * g0[e];           // gU2
+        const TW w0 = v[e].re * s_u, u0 = v[e].im * s_u, gz = r0[e], w1 = r1[e], w2 = r2[e];
+        const TW gu0 = w1 * gz - w2 * g1[e];     // gU0 = w1 g2 - w2 g1
+        const TW gu1 = w2 * g0[e] - w0 * gz;     // gU1 = w2 g0 - w0 g2
+        r1[e] = w0 * g1[e] - w1 * g0[e];         // gU2 = w0 g1 - w1 g0
+        r3[e] = u0;
         v[e] = {gu0, gu1};
     }
     __syncthreads();
     zfft<L, -1, T, NT, RL>(v, lds, ltw, tid);
     __syncthreads();
     z_split_store<L, T, NT, RL, E, HOUT>(v, lds, tid, line0, lines, hp, out + 0 * fs_out, out + 1 * fs_out, s_o);
+    // live: g0, g1, g2 = r0, gU2 = r1, U0 = r3
+    inverse(1, 2, one, one);  // (uy, uz): gW complete
 #pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = {r0[e], r3[e]};
+    for (int e = 0; e < E; ++e) {
+        const TW u1 = v[e].re * s_u, u2 = v[e].im * s_u, u0 = r3[e], gz = r0[e];
+        const TW gw0 = g1[e] * u2 - gz * u1;  // gW0 = g1 U2 - g2 U1
+        const TW gw1 = gz * u0 - g0[e] * u2;  // gW1 = g2 U0 - g0 U2
+        r3[e] = g0[e] * u1 - g1[e] * u0;      // gW2 = g0 U1 - g1 U0
+        v[e] = {gw0, gw1};
+    }
+    __syncthreads();
+    zfft<L, -1, T, NT, RL>(v, lds, ltw, tid);
+    __syncthreads();
+    z_split_store<L, T, NT, RL, E, HOUT>(v, lds, tid, line0, lines, hp, out + 3 * fs_out, out + 4 * fs_out, s_o);
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = {r1[e], r3[e]};
     zfft<L, -1, T, NT, RL>(v, lds, ltw, tid);
     __syncthreads();
     z_split_store<L, T, NT, RL, E, HOUT>(v, lds, tid, line0, lines, hp, out + 2 * fs_out, out + 5 * fs_out, s_o);
