@@ -328,6 +328,7 @@ struct vk_solver {
     DevBuf tab;             // ETD table, 7 x (qmax+1)
     int qmax;
     DevBuf sc;              // Scalars
+    DevBuf maxslots;        // per-block maxima of the max pass (kMaxSlots, kSlotStride apart)
     DevBuf tw_n, tw_m;      // work-precision twiddles
     DevBuf partials, spec;  // diagnostics
     int nblocks_diag, nshells;
@@ -430,13 +431,16 @@ int max_pass(vk_solver *s, bool omega, hipStream_t st = nullptr, void *t1 = null
     if (!st) st = s->st;
     if (!t1) t1 = s->A.p;
     if (!t2) t2 = s->B.p;
-    VK_TRY(launch_fill_u64(&s->dsc()->umax2_bits, 0ull, 1, st));
+    VK_TRY(launch_fill_u64(static_cast<unsigned long long *>(s->maxslots.p), 0ull, (long long)kMaxSlots * kSlotStride,
+                           st));
     Strided g1 = make_strided((int)NH, Hp, H, (int)NH, 0, NH, s->S, Hp, NH, Hp, NNH);
     VK_TRY(launch_inv(omega ? INV_U3DX : INV_U3, N, N, s->wp, s->wp, s->u.p, t1, s->tw_n.p, g1, N, s->ks, 1.0, st));
     Strided g2 = make_strided((int)NH, Hp, H, (int)NH, 0, NH, NNH, Hp, NH, Hp, NNH);
     VK_TRY(launch_inv(omega ? INV_OMEGA : INV_U3, N, N, s->wp, s->wp, t1, t2, s->tw_n.p, g2, N, s->ks, 1.0, st));
     const double nt = (double)N * N * N;
-    VK_TRY(launch_z_max(N, s->wp, t2, (long long)N * N, Hp, NNH, 1.0 / nt, s->tw_n.p, s->dsc(), st));
+    VK_TRY(launch_z_max(N, s->wp, t2, (long long)N * N, Hp, NNH, 1.0 / nt, s->tw_n.p,
+                        static_cast<unsigned long long *>(s->maxslots.p), st));
+    VK_TRY(launch_max_reduce(static_cast<const unsigned long long *>(s->maxslots.p), s->dsc(), st));
     return 0;
 }
 
@@ -584,6 +588,7 @@ int vk_solver_create(int n, double l, double nu, int precision, vk_solver **out)
         (r = dev_alloc(s->B, b_elems * s->wbytes(), s->total)) ||
         (r = dev_alloc(s->tab, (size_t)ETD_REC * (s->qmax + 1) * sizeof(double), s->total)) ||
         (r = dev_alloc(s->sc, sizeof(Scalars), s->total)) ||
+        (r = dev_alloc(s->maxslots, (size_t)kMaxSlots * kSlotStride * sizeof(unsigned long long), s->total)) ||
         (r = upload_twiddles(s->tw_n, n, s->wp, s->total)) ||
         (r = upload_twiddles(s->tw_m, s->m, s->wp, s->total)) ||
         (r = dev_alloc(s->partials, (size_t)s->nblocks_diag * (4 + s->nshells) * sizeof(double), s->total)) ||
@@ -619,7 +624,7 @@ int vk_solver_destroy(vk_solver *s) {
     if (s->st2) (void)hipStreamSynchronize(s->st2);
     for (DevBuf *b : {&s->u, &s->x, &s->n1, &s->sacc, &s->A, &s->B, &s->tab, &s->sc, &s->tw_n, &s->tw_m,
                       &s->partials, &s->spec, &s->staging, &s->sa_, &s->sb_, &s->sc_, &s->lam, &s->gu, &s->g1p,
-                      &s->cot, &s->WA, &s->WB, &s->C1, &s->C2})
+                      &s->cot, &s->WA, &s->WB, &s->C1, &s->C2, &s->maxslots})
         dev_free(*b);
     for (DevBuf &b : s->snaps) dev_free(b);
     for (DevBuf &b : s->stage_snaps) dev_free(b);

@@ -44,11 +44,26 @@ int launch_z_vjp(int L, Prec wp, const void *in, void *out, long long lines, int
     return 2;
 }
 #define VK_CASE_ZMAX(LEN) \
-    case LEN: return launch_z_max_L<LEN>(wp, in, lines, hp, fs, s_in, tw, sc, st);
+    case LEN: return launch_z_max_L<LEN>(wp, in, lines, hp, fs, s_in, tw, slots, st);
 int launch_z_max(int L, Prec wp, const void *in, long long lines, int hp, long long fs, double s_in,
-                 const void *tw, Scalars *sc, hipStream_t st) {
+                 const void *tw, unsigned long long *slots, hipStream_t st) {
     switch (L) { VK_FOR_EACH_LENGTH(VK_CASE_ZMAX) }
     return 2;
+}
+// maximum of the slots' bit patterns (non-negative doubles order like their
+// bits) into sc->umax2_bits
+__global__ void k_max_reduce(const unsigned long long *__restrict__ slots, Scalars *sc) {
+    unsigned long long m = slots[threadIdx.x * kSlotStride];
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(m, off);
+        m = o > m ? o : m;
+    }
+    if (threadIdx.x == 0) sc->umax2_bits = m;
+}
+int launch_max_reduce(const unsigned long long *slots, Scalars *sc, hipStream_t st) {
+    static_assert(kMaxSlots == 64, "one wave reduces the slots");
+    hipLaunchKernelGGL(k_max_reduce, dim3(1), dim3(kMaxSlots), 0, st, slots, sc);
+    return hipGetLastError() == hipSuccess ? 0 : 3;
 }
 #define VK_CASE_ZC2R(LEN) \
     case LEN: return launch_z_c2r_L<LEN>(in, out, lines, hp, scale, tw, st);

@@ -17,8 +17,8 @@ template int launch_z_nl_L<VK_L>(Prec, const void *, void *, long long, int, lon
                                  const void *, hipStream_t);
 template int launch_z_vjp_L<VK_L>(Prec, const void *, void *, long long, int, long long, long long, double, double,
                                   const void *, hipStream_t);
-template int launch_z_max_L<VK_L>(Prec, const void *, long long, int, long long, double, const void *, Scalars *,
-                                  hipStream_t);
+template int launch_z_max_L<VK_L>(Prec, const void *, long long, int, long long, double, const void *,
+                                  unsigned long long *, hipStream_t);
 template int launch_z_c2r_L<VK_L>(const void *, double *, long long, int, double, const void *, hipStream_t);
 template int launch_z_r2c_L<VK_L>(const double *, void *, long long, int, const void *, hipStream_t);
 }  // namespace vk

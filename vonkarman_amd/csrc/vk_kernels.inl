@@ -1418,10 +1418,14 @@ __global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) __attribute__((amdgpu_waves_p
 }
 
 // C2R of three unpadded fields and the maximum of |v|^2 (CFL speed or |w|).
-// The CFL max pass: T lines of 128-B rows per block.  In one-wave blocks of 2
-// lines its single atomic per block made the 256^3 CFL pass 0.50 -> 0.76 ms.
+// The tile of the z pass: one-wave blocks of 2 lines (more for short lines),
+// so each load instruction covers 32 or more consecutive positions of each
+// line (tiles of 128-B rows across T lines read 32-B pieces of 16 lines per
+// instruction in FP32).  Each block's maximum goes to one of kMaxSlots slots
+// (block index modulo the count, one 128-B line each), and one small block
+// reduces the slots: one atomic per block on a single address serialises.
 template <int L, typename TW>
-using ZMaxGeom = Geom<L, PlanSmall<L>::type::E, TW>;
+using ZMaxGeom = Geom<L, PlanSmall<L>::type::E, TW, 2 * 2 * (int)sizeof(TW)>;
 // The standalone C2R and R2C along the contiguous axis use the z pass's tile:
 // blocks of one wave holding T = 2 transform lines (more for short lines), the
 // smaller-E plan and twiddles in LDS, so each load or store instruction covers
@@ -1433,7 +1437,8 @@ using ZLineGeom = Geom<L, PlanSmall<L>::type::E, TW, 2 * 2 * (int)sizeof(TW)>;
 template <int L, typename TW>
 __global__ __launch_bounds__((ZMaxGeom<L, TW>::NT)) void k_z_max(const cpx<TW> *__restrict__ in, long long lines,
                                                                  int hp, long long fs, TW s_in,
-                                                                 const cpx<TW> *__restrict__ tw, Scalars *sc) {
+                                                                 const cpx<TW> *__restrict__ tw,
+                                                                 unsigned long long *__restrict__ slots) {
     using P = typename PlanSmall<L>::type;
     using PL = typename P::radices;
     constexpr int E = P::E;
@@ -1488,7 +1493,7 @@ __global__ __launch_bounds__((ZMaxGeom<L, TW>::NT)) void k_z_max(const cpx<TW> *
     __syncthreads();
     if (tid == 0) {
         for (int w = 1; w < NT / 64; ++w) m = fmax(m, wmax[w]);
-        atomicMax(&sc->umax2_bits, (unsigned long long)__double_as_longlong(m));
+        atomicMax(&slots[(blockIdx.x % kMaxSlots) * kSlotStride], (unsigned long long)__double_as_longlong(m));
     }
 }
 
@@ -1776,17 +1781,17 @@ int launch_z_vjp_L(Prec wp, const void *in, void *out, long long lines, int hp, 
 
 template <int L>
 int launch_z_max_L(Prec wp, const void *in, long long lines, int hp, long long fs, double s_in, const void *tw,
-                   Scalars *sc, hipStream_t st) {
+                   unsigned long long *slots, hipStream_t st) {
     if (wp == F64) {
         using G = ZMaxGeom<L, double>;
         hipLaunchKernelGGL((k_z_max<L, double>), blocks_for<void>(lines, G::T), dim3(G::NT), 0, st,
                            reinterpret_cast<const cpx<double> *>(in), lines, hp, fs, s_in,
-                           reinterpret_cast<const cpx<double> *>(tw), sc);
+                           reinterpret_cast<const cpx<double> *>(tw), slots);
     } else {
         using G = ZMaxGeom<L, float>;
         hipLaunchKernelGGL((k_z_max<L, float>), blocks_for<void>(lines, G::T), dim3(G::NT), 0, st,
                            reinterpret_cast<const cpx<float> *>(in), lines, hp, fs, float(s_in),
-                           reinterpret_cast<const cpx<float> *>(tw), sc);
+                           reinterpret_cast<const cpx<float> *>(tw), slots);
     }
     return hipGetLastError() == hipSuccess ? 0 : 3;
 }

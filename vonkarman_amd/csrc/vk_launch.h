@@ -115,9 +115,13 @@ int launch_fwd_inv(int L, int lout, Prec work_prec, const void *in, void *aout, 
 int launch_z_nl(int L, Prec work_prec, const void *in, void *out, long long lines, int hp,
                 long long fs_in, long long fs_out, double s_in, double s_out, const void *tw,
                 hipStream_t st);
-// C2R of 3 unpadded fields and max |v|^2 into sc->umax2_bits
+// C2R of 3 unpadded fields and max |v|^2: per-block maxima into the slots
+// (kMaxSlots of them, kSlotStride apart; cleared beforehand), then
+// launch_max_reduce writes their maximum to sc->umax2_bits
+constexpr int kMaxSlots = 64, kSlotStride = 16;
 int launch_z_max(int L, Prec work_prec, const void *in, long long lines, int hp, long long fs,
-                 double s_in, const void *tw, Scalars *sc, hipStream_t st);
+                 double s_in, const void *tw, unsigned long long *slots, hipStream_t st);
+int launch_max_reduce(const unsigned long long *slots, Scalars *sc, hipStream_t st);
 // standalone single-field C2R / R2C along the contiguous axis
 int launch_z_c2r(int L, Prec prec, const void *in, double *out, long long lines, int hp,
                  double scale, const void *tw, hipStream_t st);

@@ -22,7 +22,7 @@ int launch_z_vjp_L(Prec wp, const void *in, void *out, long long lines, int hp, 
                    long long fs_out, double s_u, double s_o, const void *tw, hipStream_t st);
 template <int L>
 int launch_z_max_L(Prec wp, const void *in, long long lines, int hp, long long fs, double s_in,
-                   const void *tw, Scalars *sc, hipStream_t st);
+                   const void *tw, unsigned long long *slots, hipStream_t st);
 template <int L>
 int launch_z_c2r_L(const void *in, double *out, long long lines, int hp, double scale, const void *tw,
                    hipStream_t st);
