@@ -141,7 +141,7 @@ __global__ __launch_bounds__((InvGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
     __shared__ cpx<TW> lds[L * T];
     __shared__ cpx<TW> ltw[TWN];
     const int tid = threadIdx.x;
-    const int col0 = blockIdx.x * T;
+    const int col0 = (g.xcd ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x) * T;
     const cpx<TW> zero = {TW(0), TW(0)};
     cpx<TW> v[E];
     TwStage<TWN, NT, TW> tws;
@@ -650,7 +650,7 @@ __global__ __launch_bounds__((FwdGeom<L, MODE, TW>::NT)) __attribute__((amdgpu_w
     __shared__ cpx<TW> lds[L * T];
     __shared__ cpx<TW> ltw[TWN];
     const int tid = threadIdx.x;
-    const int col0 = blockIdx.x * T;
+    const int col0 = (g.xcd ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x) * T;
     cpx<TW> v[E];
     TwStage<TWN, NT, TW> tws;
     if constexpr (PF) twiddles_load(tws, tw, tid);
@@ -932,7 +932,7 @@ __global__ __launch_bounds__((FusedGeom<L, STG, TW>::NT)) __attribute__((amdgpu_
     __shared__ cpx<TW> lds[L * T];
     __shared__ cpx<TW> ltw[TWN];
     const int tid = threadIdx.x;
-    const int col0 = blockIdx.x * T;
+    const int col0 = (g.xcd ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x) * T;
     cpx<TW> v[E], Q[E];
     cpx<TW> r[3][K];
     TwStage<TWN, NT, TW> tws;

@@ -24,7 +24,18 @@ struct Strided {
     long long in_os, in_ls, in_fs;
     int out_inner;
     long long out_os, out_ls, out_fs;
+    // tiles assigned per XCD (xcd_tile): a geometry whose rows are not whole
+    // 128-B lines on one side has tiles sharing lines with their neighbours,
+    // which this keeps in one XCD's L2 at about the same time
+    int xcd = 0;
 };
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup b to XCD
+// b % 8).  xcd_tile maps workgroup b to tile index start(b % 8) + b / 8, so each
+// XCD runs a contiguous range of tiles in order (a bijection on [0, nb)).
+__host__ __device__ inline int xcd_tile(int b, int nb) {
+    const int x = b % 8, i = b / 8, q = nb / 8, r = nb % 8;
+    return x * q + (x < r ? x : r) + i;
+}
 
 // ETD-RK4 coefficients by integer |k|^2 = q: one record of ETD_REC doubles per
 // q, so the coefficients a mode's stage reads share one 64-B line (a mode
