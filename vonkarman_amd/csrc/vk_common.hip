@@ -271,14 +271,14 @@ __global__ void k_etd_table(double *tab, int qmax, double nu, double ks, const S
     double p1h, p2h, p3h, p1, p2, p3;
     phi123(zh, cs, p1h, p2h, p3h);
     phi123(z, cs, p1, p2, p3);
-    tab[etd_index(ETD_E, q)] = exp(z);
-    tab[etd_index(ETD_EH, q)] = exp(zh);
-    tab[etd_index(ETD_A21, q)] = p1h;
-    tab[etd_index(ETD_A41, q)] = p1;
-    tab[etd_index(ETD_B1, q)] = p1 - 3.0 * p2 + 4.0 * p3;
-    tab[etd_index(ETD_B23, q)] = 2.0 * p2 - 4.0 * p3;
-    tab[etd_index(ETD_B4, q)] = -p2 + 4.0 * p3;
-    tab[etd_index(ETD_B4, q) + 1] = 0.0;
+    const int S = qmax + 1;
+    tab[etd_index(ETD_E, q, S)] = exp(z);
+    tab[etd_index(ETD_EH, q, S)] = exp(zh);
+    tab[etd_index(ETD_A21, q, S)] = p1h;
+    tab[etd_index(ETD_A41, q, S)] = p1;
+    tab[etd_index(ETD_B1, q, S)] = p1 - 3.0 * p2 + 4.0 * p3;
+    tab[etd_index(ETD_B23, q, S)] = 2.0 * p2 - 4.0 * p3;
+    tab[etd_index(ETD_B4, q, S)] = -p2 + 4.0 * p3;
 }
 int launch_etd_table(double *tab, int qmax, double nu, double ks, const Scalars *sc, int force, hipStream_t st) {
     const unsigned nb = (unsigned)((qmax + 1 + 127) / 128);
@@ -470,7 +470,7 @@ __global__ void k_adj_prep(const cpx<double> *__restrict__ lam, cpx<double> *__r
         double coef = 1.0;
         if (use_b4) {
             const int wx = wavenum(ix, n), wy = wavenum(iy, n);
-            coef = sc->dt * tab[etd_index(ETD_B4, wx * wx + wy * wy + iz * iz)];
+            coef = sc->dt * tab[etd_index(ETD_B4, wx * wx + wy * wy + iz * iz, ts)];
         }
         cpx<double> x[3];
         for (int f = 0; f < 3; ++f) x[f] = {coef * lam[f * fs + i].re, coef * lam[f * fs + i].im};

@@ -330,6 +330,7 @@ __device__ __forceinline__ void stage_chunk(const StageArgs &sa, bool st, const 
     cpx<TS> *n1 = reinterpret_cast<cpx<TS> *>(sa.n1);
     cpx<TS> *sacc = reinterpret_cast<cpx<TS> *>(sa.sacc);
     const double *tab = sa.tab;
+    const int ts = sa.tab_stride;
     const long long fs = sa.fs;
     if constexpr (STG == ST_RAW) {
         if (st) {
@@ -387,8 +388,8 @@ __device__ __forceinline__ void stage_chunk(const StageArgs &sa, bool st, const 
         cpx<TS> uu[CH][3];
 #pragma unroll
         for (int i = 0; i < CH; ++i) {
-            eh[i] = tab[etd_index(ETD_EH, q[i])];
-            a[i] = tab[etd_index(ETD_A21, q[i])];
+            eh[i] = tab[etd_index(ETD_EH, q[i], ts)];
+            a[i] = tab[etd_index(ETD_A21, q[i], ts)];
 #pragma unroll
             for (int f = 0; f < 3; ++f) uu[i][f] = ldnt(u + f * fs + idx[i]);
         }
@@ -418,10 +419,10 @@ __device__ __forceinline__ void stage_chunk(const StageArgs &sa, bool st, const 
         cpx<TS> uu[CH][3], s2[CH][3], m1[CH][3];
 #pragma unroll
         for (int i = 0; i < CH; ++i) {
-            e[i] = tab[etd_index(ETD_E, q[i])];
-            a[i] = tab[etd_index(ETD_A41, q[i])];
-            b1[i] = tab[etd_index(ETD_B1, q[i])];
-            b23[i] = tab[etd_index(ETD_B23, q[i])];
+            e[i] = tab[etd_index(ETD_E, q[i], ts)];
+            a[i] = tab[etd_index(ETD_A41, q[i], ts)];
+            b1[i] = tab[etd_index(ETD_B1, q[i], ts)];
+            b23[i] = tab[etd_index(ETD_B23, q[i], ts)];
 #pragma unroll
             for (int f = 0; f < 3; ++f) {
                 uu[i][f] = ldnt(u + f * fs + idx[i]);
@@ -458,7 +459,7 @@ __device__ __forceinline__ void stage_chunk(const StageArgs &sa, bool st, const 
         cpx<TS> p[CH][3];
 #pragma unroll
         for (int i = 0; i < CH; ++i) {
-            b4[i] = tab[etd_index(ETD_B4, q[i])];
+            b4[i] = tab[etd_index(ETD_B4, q[i], ts)];
 #pragma unroll
             for (int f = 0; f < 3; ++f) p[i][f] = ldnt(sacc + f * fs + idx[i]);
         }
@@ -506,6 +507,7 @@ __device__ __forceinline__ void adj_chunk(const StageArgs &sa, bool st, const lo
     cpx<double> *cot = reinterpret_cast<cpx<double> *>(sa.cot);
     cpx<double> *out = reinterpret_cast<cpx<double> *>(sa.out);
     const double *tab = sa.tab;
+    const int ts = sa.tab_stride;
     if constexpr (STG == AS_RAW) {
         if (st) {
 #pragma unroll
@@ -536,10 +538,10 @@ __device__ __forceinline__ void adj_chunk(const StageArgs &sa, bool st, const lo
             cpx<double> l[CH][3];
 #pragma unroll
             for (int i = 0; i < CH; ++i) {
-                e[i] = tab[etd_index(ETD_E, q[i])];
-                a41[i] = tab[etd_index(ETD_A41, q[i])];
-                b1[i] = tab[etd_index(ETD_B1, q[i])];
-                b23[i] = tab[etd_index(ETD_B23, q[i])];
+                e[i] = tab[etd_index(ETD_E, q[i], ts)];
+                a41[i] = tab[etd_index(ETD_A41, q[i], ts)];
+                b1[i] = tab[etd_index(ETD_B1, q[i], ts)];
+                b23[i] = tab[etd_index(ETD_B23, q[i], ts)];
 #pragma unroll
                 for (int f = 0; f < 3; ++f) l[i][f] = lam[f * fs + idx[i]];
             }
@@ -560,9 +562,9 @@ __device__ __forceinline__ void adj_chunk(const StageArgs &sa, bool st, const lo
             cpx<double> l[CH][3], g[CH][3];
 #pragma unroll
             for (int i = 0; i < CH; ++i) {
-                eh[i] = tab[etd_index(ETD_EH, q[i])];
-                a31[i] = tab[etd_index(ETD_A21, q[i])];
-                b23[i] = tab[etd_index(ETD_B23, q[i])];
+                eh[i] = tab[etd_index(ETD_EH, q[i], ts)];
+                a31[i] = tab[etd_index(ETD_A21, q[i], ts)];
+                b23[i] = tab[etd_index(ETD_B23, q[i], ts)];
 #pragma unroll
                 for (int f = 0; f < 3; ++f) {
                     l[i][f] = lam[f * fs + idx[i]];
@@ -583,8 +585,8 @@ __device__ __forceinline__ void adj_chunk(const StageArgs &sa, bool st, const lo
             cpx<double> g[CH][3], p[CH][3];
 #pragma unroll
             for (int i = 0; i < CH; ++i) {
-                eh[i] = tab[etd_index(ETD_EH, q[i])];
-                a21[i] = tab[etd_index(ETD_A21, q[i])];
+                eh[i] = tab[etd_index(ETD_EH, q[i], ts)];
+                a21[i] = tab[etd_index(ETD_A21, q[i], ts)];
 #pragma unroll
                 for (int f = 0; f < 3; ++f) {
                     g[i][f] = gu[f * fs + idx[i]];

@@ -37,13 +37,16 @@ __host__ __device__ inline int xcd_tile(int b, int nb) {
     return x * q + (x < r ? x : r) + i;
 }
 
-// ETD-RK4 coefficients by integer |k|^2 = q: one record of ETD_REC doubles per
-// q, so the coefficients a mode's stage reads share one 64-B line (a mode
-// reads 1 to 4 of them; rows that are read together are adjacent and
-// 16-B aligned).
+// ETD-RK4 coefficients by integer |k|^2 = q, for S = qmax + 1 values of q,
+// in three groups by the stage that reads them: (e_half, a21) for stages 1-2,
+// (e, a41, b1, b23) for stage 3 and b4 for stage 4, each group an array of
+// per-q records (16, 32 and 8 B).  A stage's gathers then touch only its own
+// group, a quarter to a half of the table, which stays in the L2.
 enum EtdRow { ETD_EH = 0, ETD_A21 = 1, ETD_E = 2, ETD_A41 = 3, ETD_B1 = 4, ETD_B23 = 5, ETD_B4 = 6 };
-constexpr int ETD_REC = 8;
-__host__ __device__ inline long long etd_index(int row, int q) { return (long long)q * ETD_REC + row; }
+constexpr int ETD_REC = 7;  // doubles per q over the three groups
+__host__ __device__ inline long long etd_index(int row, int q, int S) {
+    return row < 2 ? (long long)q * 2 + row : row < 6 ? 2LL * S + (long long)q * 4 + (row - 2) : 6LL * S + q;
+}
 
 // Device scalars of one solver (one allocation, read by kernels in place so a
 // time step never synchronises with the host).
@@ -94,7 +97,7 @@ struct StageArgs {
     void *sacc;       // N2 (ST_2), then e u + dt (b1 N1 + b23 (N2+N3)) (ST_3), read by ST_4;
                       // RK4: k1 + 2 k2 (+ 2 k3)
     long long fs;     // field stride of the state-layout arrays
-    const double *tab;  // ETD table: (qmax+1) records of ETD_REC doubles (etd_index)
+    const double *tab;  // ETD table: ETD_REC * (qmax+1) doubles (etd_index)
     int tab_stride;     // qmax+1
     const Scalars *sc;  // dt
     // reverse sweep (AS_*, complex double in both precisions): lam = cotangent of the step output, gu = cotangent
