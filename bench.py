@@ -478,7 +478,7 @@ def _dissipation_only(s, u, nsteps, dt, nu) -> float:
 KERNEL_OF_PASS = {  # kernel-name prefixes of the FP64 passes at the padded length M
     "x_inverse": "k_inv<{M}, {N}, 2, double, double>",
     "y_inverse": "k_inv<{M}, {N}, 4, double, double>",
-    "z_cross": "k_z_nl<{M}, double>",
+    "z_cross": "k_z_nl<{M}, double>",  # k_z_nl_w<double, ...> at M = 384 (see measure_traffic)
     "y_forward": "k_fwd<{M}, {N}, 0, 3, 0, double>",
     "x_forward": "k_fwd<{M}, {N}, 1, 3, ",  # the stage instances
     "x_fused": "k_fwd_stage_inv<{M}, {N}, ",
@@ -515,6 +515,8 @@ def measure_traffic(n: int, pass_name: str, timeout: float = 150.0) -> dict:
         return {"error": "rocprofv3 not found"}
     M, N, H = 3 * n // 2, n, n // 2 + 1
     want = KERNEL_OF_PASS[pass_name].format(M=M, N=N)
+    if pass_name == "z_cross" and M == 384:  # the one-line-per-wave z pass
+        want = "k_z_nl_w<double, "
     known_read_kib = 3 * n * n * H * 16 / 1024.0
     known_write_kib = 3 * n * n * ((H + 7) // 8 * 8) * 16 / 1024.0
     vals = {}

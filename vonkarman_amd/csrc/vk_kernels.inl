@@ -1287,6 +1287,261 @@ __global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) __attribute__((amdgpu_waves_p
     }
 }
 
+// ===========================================================================
+// The z pass at L = 384 with one line per wavefront.  384 = 6 * 8 * 8, and a
+// transform moves a line between two register layouts:
+//   layout 64: lane b (all 64 lanes) holds positions 64 a + b, a = 0..5;
+//   layout 48: lane j = c + 6 d0 (48 lanes) holds positions j + 48 d1 =
+//   c + 6 d0 + 48 d1, d1 = 0..7, so its loads and stores are coalesced.
+//   Lanes 48..63 repeat lanes 0..15 and write nothing.
+// zw_64to48 is the DFT of direction D as
+//   radix-6 over a, times w_384^{D b c}; exchange; radix-8 over b1 (with
+//   b = 8 b1 + b0; lane 8 c + b0), times w_64^{D b0 d0}; exchange; radix-8
+//   over b0,
+// and zw_48to64 the same passes in reverse order, each twiddle applied before
+// its butterfly (the DFT matrix is symmetric, so both compute the DFT of
+// direction D, from one layout into the other).  The spectral side uses
+// layout 48 and the physical side layout 64: the six physical fields of the
+// cross product take 6 elements per lane.  A lane holds at most four fields
+// plus one transform at once, against 12 elements per field when 32 lanes
+// share a line, so the pass runs at more waves per SIMD; each transform makes
+// two LDS exchanges instead of three.
+// ===========================================================================
+namespace zw {
+constexpr int L = 384, HIN = L / 3 + 1;
+// exchange next to the radix-6 pass: (c, b) -> c * 64 + b.  Lanes 8 c + b0
+// read b = 8 b1 + b0: 16-B elements of 8 adjacent lanes in distinct banks.
+__device__ __forceinline__ int x1(int c, int b) { return c * 64 + b; }
+// exchange between the radix-8 passes: (c, d0, b0) -> b0 + 9 c + 54 d0.  The
+// lanes 8 c + b0 (b0 across adjacent lanes) and the lanes c + 6 d0 (entry
+// = lane mod 8) both touch distinct banks.
+__device__ __forceinline__ int x2(int c, int d0, int b0) { return b0 + 9 * c + 54 * d0; }
+constexpr int NLDS = 432;
+// position of element d1 of lane j (j < 48) in layout 48
+__device__ __forceinline__ int pos48(int j, int d1) { return j + 48 * d1; }
+}  // namespace zw
+
+// v[m] *= w^m (D < 0) or conj(w)^m (D > 0), m = 1..N-1.  The powers are
+// formed anew in every transform: w1 passes through an empty asm statement,
+// so the compiler cannot keep the powers of all transforms in registers.
+template <int D, int N, typename R>
+__device__ __forceinline__ void zw_twiddle(cpx<R> (&v)[N], cpx<R> w1) {
+    asm volatile("" : "+v"(w1.re), "+v"(w1.im));
+    cpx<R> w = w1;
+#pragma unroll
+    for (int m = 1; m < N; ++m) {
+        v[m] = D < 0 ? cmul(v[m], w) : cmulc(v[m], w);
+        if (m + 1 < N) w = cmul(w, w1);
+    }
+}
+
+// wa = W_384^b (lane b), wb = W_64^{j % 8}.  The caller synchronises before
+// its next LDS write.
+template <int D, typename R>
+__device__ __forceinline__ void zw_64to48(cpx<R> (&v6)[6], cpx<R> (&v8)[8], cpx<R> *lds, int lane, int j, cpx<R> wa,
+                                          cpx<R> wb) {
+    dft<6, D>(v6);
+    zw_twiddle<D>(v6, wa);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) lds[zw::x1(c, lane)] = v6[c];
+    __syncthreads();
+    {
+        const int c = j >> 3, b0 = j & 7;
+#pragma unroll
+        for (int b1 = 0; b1 < 8; ++b1) v8[b1] = lds[zw::x1(c, 8 * b1 + b0)];
+        dft<8, D>(v8);
+        zw_twiddle<D>(v8, wb);
+        __syncthreads();
+        if (lane < 48) {
+#pragma unroll
+            for (int d0 = 0; d0 < 8; ++d0) lds[zw::x2(c, d0, b0)] = v8[d0];
+        }
+    }
+    __syncthreads();
+    {
+        const int c = j % 6, d0 = j / 6;
+#pragma unroll
+        for (int b0 = 0; b0 < 8; ++b0) v8[b0] = lds[zw::x2(c, d0, b0)];
+        dft<8, D>(v8);
+    }
+}
+
+template <int D, typename R>
+__device__ __forceinline__ void zw_48to64(cpx<R> (&v8)[8], cpx<R> (&v6)[6], cpx<R> *lds, int lane, int j, cpx<R> wa,
+                                          cpx<R> wb) {
+    dft<8, D>(v8);
+    if (lane < 48) {
+        const int c = j % 6, d0 = j / 6;
+#pragma unroll
+        for (int b0 = 0; b0 < 8; ++b0) lds[zw::x2(c, d0, b0)] = v8[b0];
+    }
+    __syncthreads();
+    {
+        const int c = j >> 3, b0 = j & 7;
+#pragma unroll
+        for (int d0 = 0; d0 < 8; ++d0) v8[d0] = lds[zw::x2(c, d0, b0)];
+        zw_twiddle<D>(v8, wb);
+        dft<8, D>(v8);
+        __syncthreads();
+        if (lane < 48) {
+#pragma unroll
+            for (int b1 = 0; b1 < 8; ++b1) lds[zw::x1(c, 8 * b1 + b0)] = v8[b1];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 6; ++c) v6[c] = lds[zw::x1(c, lane)];
+    zw_twiddle<D>(v6, wa);
+    dft<6, D>(v6);
+}
+
+// an empty asm statement that reads v: code after it runs after v is formed
+template <typename R, int N>
+__device__ __forceinline__ void zw_after(const cpx<R> (&v)[N]) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) asm volatile("" ::"v"(v[i].re), "v"(v[i].im));
+}
+
+// pair spectrum of one line (fields fa, fb) in layout 64 (S64) or layout 48:
+// raw loads and their combination (z_pair_raw / z_pair_make per slot class)
+template <bool S64>
+struct ZwSpec {
+    static constexpr int NS = S64 ? 6 : 8, W = S64 ? 64 : 48;
+    static __device__ __forceinline__ int pos(int lane, int j, int s) { return S64 ? 64 * s + lane : zw::pos48(j, s); }
+};
+template <bool S64, typename TW>
+__device__ __forceinline__ void zw_issue(ZRaw<TW> (&r)[ZwSpec<S64>::NS], gptr<cpx<TW>> fa, gptr<cpx<TW>> fb,
+                                         long long base, int lane, int j) {
+    using SP = ZwSpec<S64>;
+    static_for<SP::NS>([&](auto sc) {
+        constexpr int s = decltype(sc)::value;
+        constexpr int cls = zpair_class<zw::L, zw::HIN>(SP::W * s, SP::W);
+        z_pair_raw<zw::L, zw::HIN, true, cls, TW>(r[s], fa, fb, base, SP::pos(lane, j, s));
+    });
+}
+template <bool S64, typename TW>
+__device__ __forceinline__ void zw_combine(cpx<TW> (&v)[ZwSpec<S64>::NS], const ZRaw<TW> (&r)[ZwSpec<S64>::NS],
+                                           int lane, int j) {
+    using SP = ZwSpec<S64>;
+    static_for<SP::NS>([&](auto sc) {
+        constexpr int s = decltype(sc)::value;
+        constexpr int cls = zpair_class<zw::L, zw::HIN>(SP::W * s, SP::W);
+        v[s] = z_pair_make<zw::L, zw::HIN, true, cls, TW>(r[s], SP::pos(lane, j, s));
+    });
+}
+
+// S64: the spectral side in layout 64 (physical in layout 48), else the
+// reverse.  PF: each next pair's loads in flight during the current pair's
+// transform.  WPE: the waves-per-SIMD floor asked of the compiler.
+template <typename TW, bool S64, bool PF, int WPE>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void k_z_nl_w(
+    const cpx<TW> *__restrict__ in, cpx<TW> *__restrict__ out, long long lines, int hp, long long fs_in,
+    long long fs_out, TW s_in, TW s_out, const cpx<TW> *__restrict__ tw) {
+    using SP = ZwSpec<S64>;
+    constexpr int NS = SP::NS, NP = S64 ? 8 : 6;  // spectral and physical slots
+    constexpr int HOUT = zw::HIN, L = zw::L;
+    __shared__ cpx<TW> lds[zw::NLDS];
+    const int lane = threadIdx.x;
+    const int j = lane < 48 ? lane : lane - 48;
+    const long long base = (long long)blockIdx.x * hp;  // one line per block, grid = lines
+    ZRaw<TW> raw[NS];
+    auto issue = [&](int q) {
+        zw_issue<S64, TW>(raw, opaque(in) + (2 * q) * fs_in, opaque(in) + (2 * q + 1) * fs_in, base, lane, j);
+    };
+    if (PF) issue(0);
+    const gptr<cpx<TW>> gtw = (gptr<cpx<TW>>)tw;
+    const cpx<TW> wa = ldg(gtw, lane), wb = ldg(gtw, 6 * (j & 7));
+    cpx<TW> vs[NS], vp[NP];
+    TW r[6][NP];
+    auto inverse = [&]() {
+        if constexpr (S64) zw_64to48<+1>(vs, vp, lds, lane, j, wa, wb);
+        else zw_48to64<+1>(vs, vp, lds, lane, j, wa, wb);
+    };
+    auto forward = [&]() {
+        if constexpr (S64) zw_48to64<-1>(vp, vs, lds, lane, j, wa, wb);
+        else zw_64to48<-1>(vp, vs, lds, lane, j, wa, wb);
+    };
+    // fields: 0 ux, 1 uy, 2 uz, 3 wx, 4 wy, 5 wz; pairs (0,1) (2,3) (4,5)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        if (!PF) issue(q);
+        zw_combine<S64, TW>(vs, raw, lane, j);
+        if (PF && q + 1 < 3) {
+            // the next pair's loads are issued only once this pair's raw
+            // loads are combined (half their registers): an empty asm
+            // statement that reads the combined values comes first, and the
+            // address of each load passes through a later one (opaque)
+            zw_after(vs);
+            issue(q + 1);
+        }
+        inverse();
+#pragma unroll
+        for (int e = 0; e < NP; ++e) {
+            r[2 * q][e] = vp[e].re * s_in;
+            r[2 * q + 1][e] = vp[e].im * s_in;
+        }
+        __syncthreads();
+    }
+    // u x w (vonkarman-fft/src/dealiased.rs:79-81)
+#pragma unroll
+    for (int e = 0; e < NP; ++e) {
+        const TW ux = r[0][e], uy = r[1][e], uz = r[2][e];
+        const TW ox = r[3][e], oy = r[4][e], oz = r[5][e];
+        r[0][e] = uy * oz - uz * oy;
+        r[1][e] = uz * ox - ux * oz;
+        r[2][e] = ux * oy - uy * ox;
+    }
+    const cpx<TW> zero = {TW(0), TW(0)};
+    // spectral slots holding kept modes: k < 192 (S64) or k < 144
+    constexpr int NK = 3;
+    // (c0 + i c1): forward, then split Z_k, Z_{L-k} into the two half spectra.
+    // Z_{L-k} of a kept k >= 1 lies in [L - 191, L - 1]: the last two slots of
+    // layout 64 (index p - 256) or the last three of layout 48 (index p - 240),
+    // exchanged through LDS; Z_{L-0} = Z_0 is k's own.
+    constexpr int PLO = S64 ? 256 : 240;
+#pragma unroll
+    for (int e = 0; e < NP; ++e) vp[e] = {r[0][e], r[1][e]};
+    forward();
+    __syncthreads();
+    if (S64 || lane < 48) {
+#pragma unroll
+        for (int s = NS - (S64 ? 2 : 3); s < NS; ++s) lds[SP::pos(lane, j, s) - PLO] = vs[s];
+    }
+    __syncthreads();
+    const TW h = TW(0.5) * s_out;
+    {
+        cpx<TW> *o0 = out + base, *o1 = out + fs_out + base;
+#pragma unroll
+        for (int s = 0; s < NK; ++s) {
+            const int k = SP::pos(lane, j, s);
+            // k >= 1 pairs with L - k >= PLO when k < L - PLO; larger k are
+            // not kept and read entry 0
+            const int li = (k >= 1 && k <= L - PLO) ? L - k - PLO : 0;
+            const cpx<TW> zk = vs[s], zl = lds[li];
+            const cpx<TW> zm = k ? zl : zk;
+            const bool kept = k < HOUT;
+            const cpx<TW> va = kept ? cpx<TW>{(zk.re + zm.re) * h, (zk.im - zm.im) * h} : zero;
+            const cpx<TW> vb = kept ? cpx<TW>{(zk.im + zm.im) * h, (zm.re - zk.re) * h} : zero;
+            if ((S64 || lane < 48) && (s < NK - 1 || k < hp)) {
+                o0[k] = va;
+                o1[k] = vb;
+            }
+        }
+    }
+    __syncthreads();
+    // c2 alone: its kept outputs are stored from registers
+#pragma unroll
+    for (int e = 0; e < NP; ++e) vp[e] = {r[2][e], TW(0)};
+    forward();
+    {
+        cpx<TW> *o2 = out + 2 * fs_out + base;
+#pragma unroll
+        for (int s = 0; s < NK; ++s) {
+            const int k = SP::pos(lane, j, s);
+            if ((S64 || lane < 48) && (s < NK - 1 || k < hp)) o2[k] = k < HOUT ? cscale(vs[s], s_out) : zero;
+        }
+    }
+}
 
 // split the forward DFT of (a + i b) held in registers into the truncated
 // half spectra of a and b, times c_k s (c_0 = 1, c_k = 2 otherwise): the
@@ -1744,6 +1999,25 @@ int launch_fwd_L(int mode, int lout, int nf, Prec wp, const void *in, void *out,
 template <int L>
 int launch_z_nl_L(Prec wp, const void *in, void *out, long long lines, int hp, long long fs_in, long long fs_out,
                   double s_in, double s_out, const void *tw, hipStream_t st) {
+    if constexpr (L == 384) {
+        // one line per wave (k_z_nl_w); measured per launch at 256^3 against
+        // k_z_nl on the same box: FP64 0.758 -> 0.659 ms with the spectral
+        // side in layout 48, no prefetch and a 3-wave floor (162 VGPRs;
+        // layout 64 spills 44 B at that floor, with prefetch 68 B, and at two
+        // waves it ran 0.671-0.699 ms); FP32 0.384 -> 0.326 ms with the
+        // spectral side in layout 64 and prefetch (96 VGPRs, five waves;
+        // without prefetch 0.347, layout 48 0.356)
+        if (wp == F64) {
+            hipLaunchKernelGGL((k_z_nl_w<double, false, false, 3>), dim3((unsigned)lines), dim3(64), 0, st,
+                               reinterpret_cast<const cpx<double> *>(in), reinterpret_cast<cpx<double> *>(out), lines,
+                               hp, fs_in, fs_out, s_in, s_out, reinterpret_cast<const cpx<double> *>(tw));
+        } else {
+            hipLaunchKernelGGL((k_z_nl_w<float, true, true, 4>), dim3((unsigned)lines), dim3(64), 0, st,
+                               reinterpret_cast<const cpx<float> *>(in), reinterpret_cast<cpx<float> *>(out), lines,
+                               hp, fs_in, fs_out, float(s_in), float(s_out), reinterpret_cast<const cpx<float> *>(tw));
+        }
+        return hipGetLastError() == hipSuccess ? 0 : 3;
+    }
     if constexpr (L % 3 == 0 && (2 * L / 3) % 2 == 0) {
         if (wp == F64) {
             using G = ZNLGeom<L, double>;
