@@ -1092,17 +1092,24 @@ int launch_fwd_inv_L(int lout, Prec wp, const void *in, void *aout, const void *
 //   p > L - HIN (mirrored):  Z = conj X_{L-p} + i conj Y_{L-p}
 //   otherwise (zero gap):    Z = 0
 // CLS classifies the whole position range of the load at compile time
-// (zpair_class); in the mixed class the loads are unconditional from a
+// (zpair_class); in the mixed classes the loads are unconditional from a
 // clamped index and the cases are selected afterwards, because a load under a
 // branch is waited for before the branch joins.
-enum { ZP_DIRECT = 0, ZP_MIRROR = 1, ZP_ZERO = 2, ZP_MIXED = 3 };
+//   ZP_DIRECT0:  stored, the edge p = 0 included (range starting at 0)
+//   ZP_DIRZERO:  stored below HIN, zero gap above (no edge, no mirror)
+//   ZP_ZEROMIR:  zero gap up to L - HIN, mirrored above (no edge)
+enum { ZP_DIRECT = 0, ZP_MIRROR = 1, ZP_ZERO = 2, ZP_MIXED = 3, ZP_DIRECT0 = 4, ZP_DIRZERO = 5, ZP_ZEROMIR = 6 };
 template <int L, int HIN>
 constexpr int zpair_class(int b, int w) {
     const int e = b + w;  // exclusive end of the range
-    const bool edge = b == 0 || (b <= L / 2 && L / 2 < e);
+    const bool half = b <= L / 2 && L / 2 < e;
+    const bool edge = b == 0 || half;
     if (!edge && b >= 1 && e <= HIN) return ZP_DIRECT;
     if (!edge && b >= HIN && b > L - HIN && e <= L) return ZP_MIRROR;
     if (b >= HIN && e - 1 <= L - HIN) return ZP_ZERO;
+    if (b == 0 && !half && e <= HIN) return ZP_DIRECT0;
+    if (!edge && b >= 1 && b < HIN && e - 1 <= L - HIN) return ZP_DIRZERO;
+    if (!edge && b >= HIN && e <= L) return ZP_ZEROMIR;
     return ZP_MIXED;
 }
 // One element of the pair spectrum in two steps: the raw loads (X from fa, Y
@@ -1115,7 +1122,12 @@ struct ZRaw {
 template <int L, int HIN, bool HASB, int CLS, typename TW>
 __device__ __forceinline__ void z_pair_raw(ZRaw<TW> &r, gptr<cpx<TW>> fa, gptr<cpx<TW>> fb, long long base, int p) {
     if constexpr (CLS != ZP_ZERO) {
-        const int k = CLS == ZP_DIRECT ? p : CLS == ZP_MIRROR ? L - p : (p < HIN ? p : (p > L - HIN ? L - p : 0));
+        int k;
+        if constexpr (CLS == ZP_DIRECT || CLS == ZP_DIRECT0) k = p;
+        else if constexpr (CLS == ZP_MIRROR) k = L - p;
+        else if constexpr (CLS == ZP_DIRZERO) k = p < HIN ? p : HIN - 1;
+        else if constexpr (CLS == ZP_ZEROMIR) k = p > L - HIN ? L - p : HIN - 1;
+        else k = p < HIN ? p : (p > L - HIN ? L - p : 0);
         r.x = ldg(fa, base + k);
         if constexpr (HASB) r.y = ldg(fb, base + k);
     }
@@ -1128,10 +1140,18 @@ __device__ __forceinline__ cpx<TW> z_pair_make(const ZRaw<TW> &r, int p) {
     } else {
         const cpx<TW> X = r.x;
         const cpx<TW> Y = HASB ? r.y : zero;
+        const cpx<TW> dir = {X.re - Y.im, X.im + Y.re}, mir = {X.re + Y.im, Y.re - X.im};
         if constexpr (CLS == ZP_DIRECT) {
-            return {X.re - Y.im, X.im + Y.re};
+            return dir;
         } else if constexpr (CLS == ZP_MIRROR) {
-            return {X.re + Y.im, Y.re - X.im};
+            return mir;
+        } else if constexpr (CLS == ZP_DIRECT0) {
+            // at p = 0 the imaginary parts are dropped: Z = Re X + i Re Y
+            return p == 0 ? cpx<TW>{X.re, Y.re} : dir;
+        } else if constexpr (CLS == ZP_DIRZERO) {
+            return p < HIN ? dir : zero;
+        } else if constexpr (CLS == ZP_ZEROMIR) {
+            return p > L - HIN ? mir : zero;
         } else {
             const bool lo = p < HIN, hi = !lo && p > L - HIN, edge = p == 0 || 2 * p == L;
             const TW xi = edge ? TW(0) : X.im, yi = edge ? TW(0) : Y.im;
@@ -1432,89 +1452,51 @@ __device__ __forceinline__ void zw_combine(cpx<TW> (&v)[ZwSpec<S64>::NS], const 
 
 // S64: the spectral side in layout 64 (physical in layout 48), else the
 // reverse.  PF: each next pair's loads in flight during the current pair's
-// transform.  WPE: the waves-per-SIMD floor asked of the compiler.
-template <typename TW, bool S64, bool PF, int WPE>
+// transform.  WPE: the waves-per-SIMD floor asked of the compiler.  LPB lines
+// per block (1 or 2): with two, the c2 of the first line waits in LDS and the
+// two c2 share one forward transform (9 transforms per two lines, not 10).
+template <typename TW, bool S64, bool PF, int WPE, int LPB>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void k_z_nl_w(
     const cpx<TW> *__restrict__ in, cpx<TW> *__restrict__ out, long long lines, int hp, long long fs_in,
     long long fs_out, TW s_in, TW s_out, const cpx<TW> *__restrict__ tw) {
     using SP = ZwSpec<S64>;
     constexpr int NS = SP::NS, NP = S64 ? 8 : 6;  // spectral and physical slots
     constexpr int HOUT = zw::HIN, L = zw::L;
+    static_assert(LPB == 1 || LPB == 2, "one or two lines per block");
     __shared__ cpx<TW> lds[zw::NLDS];
+    __shared__ TW park[LPB == 2 ? NP * 64 : 1];
     const int lane = threadIdx.x;
     const int j = lane < 48 ? lane : lane - 48;
-    const long long base = (long long)blockIdx.x * hp;  // one line per block, grid = lines
-    ZRaw<TW> raw[NS];
-    auto issue = [&](int q) {
-        zw_issue<S64, TW>(raw, opaque(in) + (2 * q) * fs_in, opaque(in) + (2 * q + 1) * fs_in, base, lane, j);
-    };
-    if (PF) issue(0);
+    const bool wr = S64 || lane < 48;  // lanes that own spectral slots
     const gptr<cpx<TW>> gtw = (gptr<cpx<TW>>)tw;
     const cpx<TW> wa = ldg(gtw, lane), wb = ldg(gtw, 6 * (j & 7));
+    const cpx<TW> zero = {TW(0), TW(0)};
+    const TW h = TW(0.5) * s_out;
     cpx<TW> vs[NS], vp[NP];
-    TW r[6][NP];
-    auto inverse = [&]() {
-        if constexpr (S64) zw_64to48<+1>(vs, vp, lds, lane, j, wa, wb);
-        else zw_48to64<+1>(vs, vp, lds, lane, j, wa, wb);
-    };
     auto forward = [&]() {
         if constexpr (S64) zw_48to64<-1>(vp, vs, lds, lane, j, wa, wb);
         else zw_64to48<-1>(vp, vs, lds, lane, j, wa, wb);
     };
-    // fields: 0 ux, 1 uy, 2 uz, 3 wx, 4 wy, 5 wz; pairs (0,1) (2,3) (4,5)
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-        if (!PF) issue(q);
-        zw_combine<S64, TW>(vs, raw, lane, j);
-        if (PF && q + 1 < 3) {
-            // the next pair's loads are issued only once this pair's raw
-            // loads are combined (half their registers): an empty asm
-            // statement that reads the combined values comes first, and the
-            // address of each load passes through a later one (opaque)
-            zw_after(vs);
-            issue(q + 1);
-        }
-        inverse();
-#pragma unroll
-        for (int e = 0; e < NP; ++e) {
-            r[2 * q][e] = vp[e].re * s_in;
-            r[2 * q + 1][e] = vp[e].im * s_in;
-        }
-        __syncthreads();
-    }
-    // u x w (vonkarman-fft/src/dealiased.rs:79-81)
-#pragma unroll
-    for (int e = 0; e < NP; ++e) {
-        const TW ux = r[0][e], uy = r[1][e], uz = r[2][e];
-        const TW ox = r[3][e], oy = r[4][e], oz = r[5][e];
-        r[0][e] = uy * oz - uz * oy;
-        r[1][e] = uz * ox - ux * oz;
-        r[2][e] = ux * oy - uy * ox;
-    }
-    const cpx<TW> zero = {TW(0), TW(0)};
     // spectral slots holding kept modes: k < 192 (S64) or k < 144
     constexpr int NK = 3;
-    // (c0 + i c1): forward, then split Z_k, Z_{L-k} into the two half spectra.
-    // Z_{L-k} of a kept k >= 1 lies in [L - 191, L - 1]: the last two slots of
-    // layout 64 (index p - 256) or the last three of layout 48 (index p - 240),
-    // exchanged through LDS; Z_{L-0} = Z_0 is k's own.
+    // Forward DFT of (a + i b) in vp, split into the half spectra of a and b
+    // (stored at oa, ob).  Z_{L-k} of a kept k >= 1 lies in [L - 191, L - 1]:
+    // the last two slots of layout 64 (index p - 256) or the last three of
+    // layout 48 (index p - 240), exchanged through LDS; Z_{L-0} = Z_0 is k's
+    // own.  Lines are written whole, entries [HOUT, hp) as zeros.
     constexpr int PLO = S64 ? 256 : 240;
+    auto forward_split = [&](cpx<TW> *oa, cpx<TW> *ob) {
+        forward();
+        __syncthreads();
+        if (wr) {
 #pragma unroll
-    for (int e = 0; e < NP; ++e) vp[e] = {r[0][e], r[1][e]};
-    forward();
-    __syncthreads();
-    if (S64 || lane < 48) {
-#pragma unroll
-        for (int s = NS - (S64 ? 2 : 3); s < NS; ++s) lds[SP::pos(lane, j, s) - PLO] = vs[s];
-    }
-    __syncthreads();
-    const TW h = TW(0.5) * s_out;
-    {
-        cpx<TW> *o0 = out + base, *o1 = out + fs_out + base;
+            for (int s = NS - (S64 ? 2 : 3); s < NS; ++s) lds[SP::pos(lane, j, s) - PLO] = vs[s];
+        }
+        __syncthreads();
 #pragma unroll
         for (int s = 0; s < NK; ++s) {
             const int k = SP::pos(lane, j, s);
-            // k >= 1 pairs with L - k >= PLO when k < L - PLO; larger k are
+            // k >= 1 pairs with L - k >= PLO when k <= L - PLO; larger k are
             // not kept and read entry 0
             const int li = (k >= 1 && k <= L - PLO) ? L - k - PLO : 0;
             const cpx<TW> zk = vs[s], zl = lds[li];
@@ -1522,23 +1504,82 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
             const bool kept = k < HOUT;
             const cpx<TW> va = kept ? cpx<TW>{(zk.re + zm.re) * h, (zk.im - zm.im) * h} : zero;
             const cpx<TW> vb = kept ? cpx<TW>{(zk.im + zm.im) * h, (zm.re - zk.re) * h} : zero;
-            if ((S64 || lane < 48) && (s < NK - 1 || k < hp)) {
-                o0[k] = va;
-                o1[k] = vb;
+            if (wr && (s < NK - 1 || k < hp)) {
+                oa[k] = va;
+                ob[k] = vb;
             }
         }
-    }
-    __syncthreads();
-    // c2 alone: its kept outputs are stored from registers
+        __syncthreads();
+    };
+    ZRaw<TW> raw[NS];
+    // grid = lines / LPB (the launcher checks that LPB divides lines)
+    auto issue = [&](int l, int q) {
+        const long long base = ((long long)blockIdx.x * LPB + l) * hp;
+        zw_issue<S64, TW>(raw, opaque(in) + (2 * q) * fs_in, opaque(in) + (2 * q + 1) * fs_in, base, lane, j);
+    };
 #pragma unroll
-    for (int e = 0; e < NP; ++e) vp[e] = {r[2][e], TW(0)};
-    forward();
-    {
-        cpx<TW> *o2 = out + 2 * fs_out + base;
+    for (int l = 0; l < LPB; ++l) {
+        const long long base = ((long long)blockIdx.x * LPB + l) * hp;
+        if (PF) issue(l, 0);
+        TW r[6][NP];
+        // fields: 0 ux, 1 uy, 2 uz, 3 wx, 4 wy, 5 wz; pairs (0,1) (2,3) (4,5)
 #pragma unroll
-        for (int s = 0; s < NK; ++s) {
-            const int k = SP::pos(lane, j, s);
-            if ((S64 || lane < 48) && (s < NK - 1 || k < hp)) o2[k] = k < HOUT ? cscale(vs[s], s_out) : zero;
+        for (int q = 0; q < 3; ++q) {
+            if (!PF) issue(l, q);
+            zw_combine<S64, TW>(vs, raw, lane, j);
+            if (PF && q + 1 < 3) {
+                // the next pair's loads are issued only once this pair's raw
+                // loads are combined (half their registers): an empty asm
+                // statement that reads the combined values comes first, and
+                // the address of each load passes through a later one (opaque)
+                zw_after(vs);
+                issue(l, q + 1);
+            }
+            if constexpr (S64) zw_64to48<+1>(vs, vp, lds, lane, j, wa, wb);
+            else zw_48to64<+1>(vs, vp, lds, lane, j, wa, wb);
+#pragma unroll
+            for (int e = 0; e < NP; ++e) {
+                r[2 * q][e] = vp[e].re * s_in;
+                r[2 * q + 1][e] = vp[e].im * s_in;
+            }
+            __syncthreads();
+        }
+        // u x w (vonkarman-fft/src/dealiased.rs:79-81)
+#pragma unroll
+        for (int e = 0; e < NP; ++e) {
+            const TW ux = r[0][e], uy = r[1][e], uz = r[2][e];
+            const TW ox = r[3][e], oy = r[4][e], oz = r[5][e];
+            r[0][e] = uy * oz - uz * oy;
+            r[1][e] = uz * ox - ux * oz;
+            r[2][e] = ux * oy - uy * ox;
+        }
+        if (LPB == 2 && l == 0) {
+            // c2 of the first line waits in LDS for the second line's
+#pragma unroll
+            for (int e = 0; e < NP; ++e) park[e * 64 + lane] = r[2][e];
+        }
+        // (c0 + i c1)
+#pragma unroll
+        for (int e = 0; e < NP; ++e) vp[e] = {r[0][e], r[1][e]};
+        forward_split(out + base, out + fs_out + base);
+        if constexpr (LPB == 2) {
+            if (l == 1) {
+                // (c2 of the first line) + i (c2 of this line)
+#pragma unroll
+                for (int e = 0; e < NP; ++e) vp[e] = {park[e * 64 + lane], r[2][e]};
+                forward_split(out + 2 * fs_out + base - hp, out + 2 * fs_out + base);
+            }
+        } else {
+            // c2 alone: its kept outputs are stored from registers
+#pragma unroll
+            for (int e = 0; e < NP; ++e) vp[e] = {r[2][e], TW(0)};
+            forward();
+            cpx<TW> *o2 = out + 2 * fs_out + base;
+#pragma unroll
+            for (int s = 0; s < NK; ++s) {
+                const int k = SP::pos(lane, j, s);
+                if (wr && (s < NK - 1 || k < hp)) o2[k] = k < HOUT ? cscale(vs[s], s_out) : zero;
+            }
         }
     }
 }
@@ -2001,18 +2042,27 @@ int launch_z_nl_L(Prec wp, const void *in, void *out, long long lines, int hp, l
                   double s_in, double s_out, const void *tw, hipStream_t st) {
     if constexpr (L == 384) {
         // one line per wave (k_z_nl_w); measured per launch at 256^3 against
-        // k_z_nl on the same box: FP64 0.758 -> 0.659 ms with the spectral
-        // side in layout 48, no prefetch and a 3-wave floor (162 VGPRs;
-        // layout 64 spills 44 B at that floor, with prefetch 68 B, and at two
-        // waves it ran 0.671-0.699 ms); FP32 0.384 -> 0.326 ms with the
-        // spectral side in layout 64 and prefetch (96 VGPRs, five waves;
-        // without prefetch 0.347, layout 48 0.356)
+        // k_z_nl (0.758 / 0.384 ms FP64 / FP32) on the same boxes:
+        //   FP64: spectral side in layout 48, no prefetch, a 3-wave floor:
+        //   0.659 ms with one line per block (162 VGPRs), 0.61-0.625 with two
+        //   (c2 paired; 166 VGPRs).  Layout 64 spills 44 B at that floor
+        //   (0.742 ms), prefetch at two waves ran 0.63-0.70, and issuing the
+        //   second line's loads during the first line's forward transforms
+        //   measured the same.
+        //   FP32: spectral side in layout 64 with prefetch, one line per block
+        //   (92 VGPRs, five waves): 0.326 ms; two lines per block 0.331,
+        //   without prefetch 0.347, layout 48 0.356.
         if (wp == F64) {
-            hipLaunchKernelGGL((k_z_nl_w<double, false, false, 3>), dim3((unsigned)lines), dim3(64), 0, st,
-                               reinterpret_cast<const cpx<double> *>(in), reinterpret_cast<cpx<double> *>(out), lines,
-                               hp, fs_in, fs_out, s_in, s_out, reinterpret_cast<const cpx<double> *>(tw));
+            if (lines % 2 == 0)
+                hipLaunchKernelGGL((k_z_nl_w<double, false, false, 3, 2>), dim3((unsigned)(lines / 2)), dim3(64), 0,
+                                   st, reinterpret_cast<const cpx<double> *>(in), reinterpret_cast<cpx<double> *>(out),
+                                   lines, hp, fs_in, fs_out, s_in, s_out, reinterpret_cast<const cpx<double> *>(tw));
+            else
+                hipLaunchKernelGGL((k_z_nl_w<double, false, false, 3, 1>), dim3((unsigned)lines), dim3(64), 0, st,
+                                   reinterpret_cast<const cpx<double> *>(in), reinterpret_cast<cpx<double> *>(out),
+                                   lines, hp, fs_in, fs_out, s_in, s_out, reinterpret_cast<const cpx<double> *>(tw));
         } else {
-            hipLaunchKernelGGL((k_z_nl_w<float, true, true, 4>), dim3((unsigned)lines), dim3(64), 0, st,
+            hipLaunchKernelGGL((k_z_nl_w<float, true, true, 4, 1>), dim3((unsigned)lines), dim3(64), 0, st,
                                reinterpret_cast<const cpx<float> *>(in), reinterpret_cast<cpx<float> *>(out), lines,
                                hp, fs_in, fs_out, float(s_in), float(s_out), reinterpret_cast<const cpx<float> *>(tw));
         }
