@@ -160,18 +160,18 @@ def run_gpu(args):
     return n, out, timing, fft, {"energy_rel_err": abs(emx - e64) / e64, "field_rel_l2": field_rel}
 
 
-def adjoint_cost(n: int, nsteps: int = 8) -> dict:
+def adjoint_cost(n: int, nsteps: int = 8, precision: str = "f64") -> dict:
     """Wall time of the discrete adjoint against the forward steps it
-    differentiates: FP64, fixed dt (the CFL dt of the IC), J = time-integrated
-    dissipation.  With nsteps snapshots every state is stored (one forward
-    sweep); with 2 the binomial schedule recomputes."""
+    differentiates: FP64 or mixed, fixed dt (the CFL dt of the IC), J =
+    time-integrated dissipation.  With nsteps snapshots every state is stored
+    (one forward sweep); with 2 the binomial schedule recomputes."""
     import ctypes
 
     from vonkarman_amd import GridSpec, IcType, Periodic3D, _lib
     from vonkarman_amd import ic as vic
 
     u0 = vic.von_karman_isotropic(n, k0=4.0, kd=64.0, energy=0.5, seed=42)
-    s = Periodic3D(GridSpec.cubic(n), 1e-3, IcType.Spectral(u0))
+    s = Periodic3D(GridSpec.cubic(n), 1e-3, IcType.Spectral(u0), precision=precision)
     s.set_fixed_dt(s.dt())
     s.steps(1)  # warm (kernels loaded, ETD table built)
     s.set_u_hat(u0)
@@ -181,7 +181,8 @@ def adjoint_cost(n: int, nsteps: int = 8) -> dict:
     t_fwd = time.perf_counter() - t0
     s.set_u_hat(u0)
     s.set_clock(0.0, 0)
-    res = {"n": n, "steps": nsteps, "dt": s.dt(), "objective": "dissipation", "forward_s": t_fwd}
+    res = {"n": n, "precision": precision, "steps": nsteps, "dt": s.dt(), "objective": "dissipation",
+           "forward_s": t_fwd}
     L = _lib.lib()
     for snaps in (1 + nsteps, 2):
         J = ctypes.c_double(0.0)
@@ -202,7 +203,7 @@ def adjoint_cost(n: int, nsteps: int = 8) -> dict:
                     "reverse_step_ms": 1e3 * rev_step, "reverse_step_over_forward_step": rev_step / step_fwd}
     res["forward_step_ms"] = 1e3 * t_fwd / nsteps
     res["device_bytes"] = s.device_bytes()
-    print(f"[bench] adjoint: {nsteps} steps, reverse step / forward step "
+    print(f"[bench] adjoint {precision}: {nsteps} steps, reverse step / forward step "
           f"{res['all_states']['reverse_step_over_forward_step']:.2f}, gradient / forward trajectory "
           f"{res['all_states']['gradient_over_forward_trajectory']:.2f} "
           f"(two snapshots {res['two_snapshots']['gradient_over_forward_trajectory']:.2f})", file=sys.stderr,
@@ -485,14 +486,14 @@ KERNEL_OF_PASS = {  # kernel-name prefixes of the FP64 passes at the padded leng
 }
 
 
-def pmc_child(n: int) -> None:
-    """Workload of the counter runs: FP64 solver set-up (one state repack of
-    known size) and one adaptive step."""
+def pmc_child(n: int, precision: str = "f64") -> None:
+    """Workload of the counter runs: solver set-up (one state repack of known
+    size) and one adaptive step, FP64 or mixed."""
     from vonkarman_amd import GridSpec, IcType, Periodic3D
     from vonkarman_amd import ic as vic
 
     u0 = vic.von_karman_isotropic(n, seed=42)
-    s = Periodic3D(GridSpec.cubic(n), 1e-3, IcType.Spectral(u0))
+    s = Periodic3D(GridSpec.cubic(n), 1e-3, IcType.Spectral(u0), precision=precision)
     s.steps(1)
 
 
@@ -737,9 +738,10 @@ def main():
                     help="step: the throughput line (default); the others run the accuracy/adjoint configurations "
                          "and print their own JSON line")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--pmc-precision", default="f64", choices=["f64", "mixed"], help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pmc_child:
-        pmc_child(args.n)
+        pmc_child(args.n, args.pmc_precision)
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -757,10 +759,14 @@ def main():
     sps_mixed = args.steps / out["mixed"]["sec"]
     pb = pass_bytes(n, 8)
     avg = timing["f64"]["avg_ms"]
-    dom = max(avg, key=lambda k: avg[k])
+    # the dominant pass: the most time per step (y and z passes run once per
+    # nonlinear evaluation, four times a step; the x passes as counted)
+    per_step = {k: avg[k] * timing["f64"]["launches_per_step"].get(k, 4.0) for k in avg}
+    dom = max(per_step, key=lambda k: per_step[k])
     achieved = pb[dom] / (avg[dom] * 1e-3) / 1e9
     traffic = None if args.no_traffic else measure_traffic(n, dom)
     adj = adjoint_cost(n) if not args.no_adjoint else None
+    adj_mixed = adjoint_cost(n, precision="mixed") if not args.no_adjoint else None
     cfg = None if args.no_configs else config_evidence(n)
     cpu = None if args.no_cpu else cpu_baseline(n)
     sb = step_bytes(n, 8)
@@ -783,7 +789,7 @@ def main():
                    "nu": 1e-3, "cfl_safety": 0.5, "spectrum_interval": 10},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": (traffic or {}).get("bytes"),
-                     "bytes_per_launch": pb[dom], "avg_launch_ms": avg[dom],
+                     "bytes_per_launch": pb[dom], "avg_launch_ms": avg[dom], "ms_per_step": per_step[dom],
                      "traffic_detail": traffic},
         "cpu_baseline": cpu,
         "step_hbm_frac": sb * sps / (HBM_PEAK_GBS * 1e9),
@@ -796,6 +802,7 @@ def main():
                   "steps_compared": args.warmup + args.steps + timing["f64"]["steps"]},
         "fft_c2r": fft,
         "adjoint": adj,
+        "adjoint_mixed": adj_mixed,
         "configs": cfg,
     }
     print(json.dumps(line), flush=True)

@@ -398,12 +398,17 @@ int nl_eval(vk_solver *s, const void *src, StageArgs &sa, vk_timing *t, hipEvent
     };
     if (int r = mark(0)) return r;
     // P1: x inverse with padding: u[3][N][N][Hp] -> A[5][N][M][Hp] (y-major)
+    // P1 and P2 run their tiles per XCD (xcd_tile): at 256^3 the x inverse
+    // 0.298 -> 0.291 ms and the y inverse 0.624 -> 0.601 ms (FP32 0.325 ->
+    // 0.315), same box; the y forward measured 1-2% slower with it
     Strided g1 = make_strided((int)NH, Hp, H, (int)NH, 0, NH, s->S, Hp, MH, Hp, NMH);
+    g1.xcd = 1;
     if (!skip_p1)
         VK_TRY(launch_inv(INV_U3DX, M, N, s->wp, s->wp, src, s->A.p, s->tw_m.p, g1, N, s->ks, 1.0, s->st));
     if (int r = mark(1)) return r;
     // P2: y inverse with padding, curl completed: A -> B[6][M][M][Hp]
     Strided g2 = make_strided((int)MH, Hp, H, (int)MH, 0, MH, NMH, Hp, MH, Hp, MMH);
+    g2.xcd = 1;
     void *yb = bbuf ? bbuf : s->B.p;
     VK_TRY(launch_inv(INV_UOMEGA, M, N, s->wp, s->wp, s->A.p, yb, s->tw_m.p, g2, N, s->ks, 1.0, s->st));
     if (int r = mark(2)) return r;

@@ -1329,9 +1329,10 @@ __global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) __attribute__((amdgpu_waves_p
 // ===========================================================================
 namespace zw {
 constexpr int L = 384, HIN = L / 3 + 1;
-// exchange next to the radix-6 pass: (c, b) -> c * 64 + b.  Lanes 8 c + b0
-// read b = 8 b1 + b0: 16-B elements of 8 adjacent lanes in distinct banks.
-__device__ __forceinline__ int x1(int c, int b) { return c * 64 + b; }
+// exchange next to the radix-6 pass: (c, b) -> c * 72 + b.  Lanes 8 c + b0
+// read b = 8 b1 + b0: 16-B elements of 8 adjacent lanes, and 8-B elements of
+// 16 adjacent lanes (c * 72 = 8 mod 16), fall in distinct banks.
+__device__ __forceinline__ int x1(int c, int b) { return c * 72 + b; }
 // exchange between the radix-8 passes: (c, d0, b0) -> b0 + 9 c + 54 d0.  The
 // lanes 8 c + b0 (b0 across adjacent lanes) and the lanes c + 6 d0 (entry
 // = lane mod 8) both touch distinct banks.
@@ -1429,24 +1430,24 @@ struct ZwSpec {
     static constexpr int NS = S64 ? 6 : 8, W = S64 ? 64 : 48;
     static __device__ __forceinline__ int pos(int lane, int j, int s) { return S64 ? 64 * s + lane : zw::pos48(j, s); }
 };
-template <bool S64, typename TW>
+template <bool S64, typename TW, bool HASB = true>
 __device__ __forceinline__ void zw_issue(ZRaw<TW> (&r)[ZwSpec<S64>::NS], gptr<cpx<TW>> fa, gptr<cpx<TW>> fb,
                                          long long base, int lane, int j) {
     using SP = ZwSpec<S64>;
     static_for<SP::NS>([&](auto sc) {
         constexpr int s = decltype(sc)::value;
         constexpr int cls = zpair_class<zw::L, zw::HIN>(SP::W * s, SP::W);
-        z_pair_raw<zw::L, zw::HIN, true, cls, TW>(r[s], fa, fb, base, SP::pos(lane, j, s));
+        z_pair_raw<zw::L, zw::HIN, HASB, cls, TW>(r[s], fa, fb, base, SP::pos(lane, j, s));
     });
 }
-template <bool S64, typename TW>
+template <bool S64, typename TW, bool HASB = true>
 __device__ __forceinline__ void zw_combine(cpx<TW> (&v)[ZwSpec<S64>::NS], const ZRaw<TW> (&r)[ZwSpec<S64>::NS],
                                            int lane, int j) {
     using SP = ZwSpec<S64>;
     static_for<SP::NS>([&](auto sc) {
         constexpr int s = decltype(sc)::value;
         constexpr int cls = zpair_class<zw::L, zw::HIN>(SP::W * s, SP::W);
-        v[s] = z_pair_make<zw::L, zw::HIN, true, cls, TW>(r[s], SP::pos(lane, j, s));
+        v[s] = z_pair_make<zw::L, zw::HIN, HASB, cls, TW>(r[s], SP::pos(lane, j, s));
     });
 }
 
@@ -1713,6 +1714,126 @@ __global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) __attribute__((amdgpu_waves_p
     zfft<L, -1, T, NT, RL>(v, lds, ltw, tid);
     __syncthreads();
     z_split_store<L, T, NT, RL, E, HOUT>(v, lds, tid, line0, lines, hp, out + 2 * fs_out, out + 5 * fs_out, s_o);
+}
+
+// The VJP z pass at L = 384 with one line per wavefront (the layouts of
+// k_z_nl_w): the same fields, transforms and output as k_z_vjp.
+template <typename TW, bool S64, int WPE>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void k_z_vjp_w(
+    const cpx<TW> *__restrict__ in, cpx<TW> *__restrict__ out, long long lines, int hp, long long fs_in,
+    long long fs_out, TW s_u, TW s_o, const cpx<TW> *__restrict__ tw) {
+    using SP = ZwSpec<S64>;
+    constexpr int NS = SP::NS, NP = S64 ? 8 : 6;  // spectral and physical slots
+    constexpr int HOUT = zw::HIN, L = zw::L;
+    __shared__ cpx<TW> lds[zw::NLDS];
+    const int lane = threadIdx.x;
+    const int j = lane < 48 ? lane : lane - 48;
+    const bool wr = S64 || lane < 48;  // lanes that own spectral slots
+    const long long base = (long long)blockIdx.x * hp;  // one line per block, grid = lines
+    const gptr<cpx<TW>> gtw = (gptr<cpx<TW>>)tw;
+    const cpx<TW> wa = ldg(gtw, lane), wb = ldg(gtw, 6 * (j & 7));
+    const cpx<TW> zero = {TW(0), TW(0)};
+    const TW one = TW(1), two = TW(2), half = TW(0.5);
+    cpx<TW> vs[NS], vp[NP];
+    TW g0[NP], g1[NP], r0[NP], r1[NP], r2[NP], r3[NP];
+    // inverse pair transform of fields (fa, fb) (fb < 0: none) with per-half
+    // DC factors: the z transpose of an R2C counts its k=0 term once where the
+    // C2R extension would count it twice
+    auto inverse = [&](int fa, int fb, TW dca, TW dcb) {
+        ZRaw<TW> raw[NS];
+        const gptr<cpx<TW>> pa = opaque(in) + fa * fs_in;
+        if (fb >= 0) {
+            zw_issue<S64, TW, true>(raw, pa, opaque(in) + fb * fs_in, base, lane, j);
+            zw_combine<S64, TW, true>(vs, raw, lane, j);
+        } else {
+            zw_issue<S64, TW, false>(raw, pa, pa, base, lane, j);
+            zw_combine<S64, TW, false>(vs, raw, lane, j);
+        }
+        // position 0 is slot 0 of lane 0 in both layouts
+        if (lane == 0) vs[0] = {vs[0].re * dca, vs[0].im * dcb};
+        if constexpr (S64) zw_64to48<+1>(vs, vp, lds, lane, j, wa, wb);
+        else zw_48to64<+1>(vs, vp, lds, lane, j, wa, wb);
+    };
+    // forward DFT of vp split into the truncated half spectra of its real and
+    // imaginary parts, times c_k s (c_0 = 1, c_k = 2): the transpose of the
+    // unnormalised C2R.  Lines are written whole, entries [HOUT, hp) as zeros.
+    constexpr int NK = 3, PLO = S64 ? 256 : 240;
+    auto forward_split = [&](cpx<TW> *oa, cpx<TW> *ob) {
+        if constexpr (S64) zw_48to64<-1>(vp, vs, lds, lane, j, wa, wb);
+        else zw_64to48<-1>(vp, vs, lds, lane, j, wa, wb);
+        __syncthreads();
+        if (wr) {
+#pragma unroll
+            for (int s = NS - (S64 ? 2 : 3); s < NS; ++s) lds[SP::pos(lane, j, s) - PLO] = vs[s];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < NK; ++s) {
+            const int k = SP::pos(lane, j, s);
+            const int li = (k >= 1 && k <= L - PLO) ? L - k - PLO : 0;
+            const cpx<TW> zk = vs[s], zl = lds[li];
+            const cpx<TW> zm = k ? zl : zk;
+            const TW h = k ? s_o : half * s_o;
+            const bool kept = k < HOUT;
+            const cpx<TW> va = kept ? cpx<TW>{(zk.re + zm.re) * h, (zk.im - zm.im) * h} : zero;
+            const cpx<TW> vb = kept ? cpx<TW>{(zk.im + zm.im) * h, (zm.re - zk.re) * h} : zero;
+            if (wr && (s < NK - 1 || k < hp)) {
+                oa[base + k] = va;
+                ob[base + k] = vb;
+            }
+        }
+        __syncthreads();
+    };
+    // Cotangent fields are paired only with cotangent fields: a complex FFT
+    // of (a + i b) leaks roundoff of the larger of a, b into the other, and u, w
+    // are many orders above g.
+    // g = R2C^T gin: C2R with the DC counted once, times 1/2
+    inverse(6, 7, two, two);
+#pragma unroll
+    for (int e = 0; e < NP; ++e) {
+        g0[e] = vp[e].re * half;
+        g1[e] = vp[e].im * half;
+    }
+    __syncthreads();
+    inverse(8, -1, two, one);  // (gz, 0)
+#pragma unroll
+    for (int e = 0; e < NP; ++e) r0[e] = vp[e].re * half;  // g2
+    __syncthreads();
+    // W before U, holding W1, W2 and then U0 rather than partial products
+    inverse(4, 5, one, one);  // (wy, wz), held
+#pragma unroll
+    for (int e = 0; e < NP; ++e) {
+        r1[e] = vp[e].re * s_u;  // W1
+        r2[e] = vp[e].im * s_u;  // W2
+    }
+    __syncthreads();
+    inverse(3, 0, one, one);  // (wx, ux): gU complete; U0 held
+#pragma unroll
+    for (int e = 0; e < NP; ++e) {
+        const TW w0 = vp[e].re * s_u, u0 = vp[e].im * s_u, gz = r0[e], w1 = r1[e], w2 = r2[e];
+        const TW gu0 = w1 * gz - w2 * g1[e];  // gU0 = w1 g2 - w2 g1
+        const TW gu1 = w2 * g0[e] - w0 * gz;  // gU1 = w2 g0 - w0 g2
+        r1[e] = w0 * g1[e] - w1 * g0[e];      // gU2 = w0 g1 - w1 g0
+        r3[e] = u0;
+        vp[e] = {gu0, gu1};
+    }
+    __syncthreads();
+    forward_split(out + 0 * fs_out, out + 1 * fs_out);
+    // live: g0, g1, g2 = r0, gU2 = r1, U0 = r3
+    inverse(1, 2, one, one);  // (uy, uz): gW complete
+#pragma unroll
+    for (int e = 0; e < NP; ++e) {
+        const TW u1 = vp[e].re * s_u, u2 = vp[e].im * s_u, u0 = r3[e], gz = r0[e];
+        const TW gw0 = g1[e] * u2 - gz * u1;  // gW0 = g1 U2 - g2 U1
+        const TW gw1 = gz * u0 - g0[e] * u2;  // gW1 = g2 U0 - g0 U2
+        r3[e] = g0[e] * u1 - g1[e] * u0;      // gW2 = g0 U1 - g1 U0
+        vp[e] = {gw0, gw1};
+    }
+    __syncthreads();
+    forward_split(out + 3 * fs_out, out + 4 * fs_out);
+#pragma unroll
+    for (int e = 0; e < NP; ++e) vp[e] = {r1[e], r3[e]};
+    forward_split(out + 2 * fs_out, out + 5 * fs_out);
 }
 
 // C2R of three unpadded fields and the maximum of |v|^2 (CFL speed or |w|).
@@ -2088,6 +2209,23 @@ int launch_z_nl_L(Prec wp, const void *in, void *out, long long lines, int hp, l
 template <int L>
 int launch_z_vjp_L(Prec wp, const void *in, void *out, long long lines, int hp, long long fs_in, long long fs_out,
                    double s_u, double s_o, const void *tw, hipStream_t st) {
+    if constexpr (L == 384) {
+        // one line per wave (k_z_vjp_w); measured at 256^3 by the adjoint's
+        // reverse step against k_z_vjp on the same box: FP64 17.8 -> 17.2 ms
+        // with the spectral side in layout 48 at two waves (196 VGPRs; a
+        // 3-wave floor spills 156 B, 108 B with W1, W2 held in LDS, and ran
+        // 18.7 ms); mixed 10.94 -> 10.09 ms with the spectral side in layout
+        // 64 (112 VGPRs, four waves; layout 48 10.33 ms)
+        if (wp == F64)
+            hipLaunchKernelGGL((k_z_vjp_w<double, false, 2>), dim3((unsigned)lines), dim3(64), 0, st,
+                               reinterpret_cast<const cpx<double> *>(in), reinterpret_cast<cpx<double> *>(out), lines,
+                               hp, fs_in, fs_out, s_u, s_o, reinterpret_cast<const cpx<double> *>(tw));
+        else
+            hipLaunchKernelGGL((k_z_vjp_w<float, true, 4>), dim3((unsigned)lines), dim3(64), 0, st,
+                               reinterpret_cast<const cpx<float> *>(in), reinterpret_cast<cpx<float> *>(out), lines,
+                               hp, fs_in, fs_out, float(s_u), float(s_o), reinterpret_cast<const cpx<float> *>(tw));
+        return hipGetLastError() == hipSuccess ? 0 : 3;
+    }
     if constexpr (L % 3 == 0 && (2 * L / 3) % 2 == 0) {
         if (wp == F64) {
             using G = ZNLGeom<L, double>;
