@@ -193,11 +193,15 @@ static int plan_r2c_kernels(vk_fft_plan *p, void *spec) {
     Strided gy = make_strided(nx * hp, hp, hz, hp, (long long)ny * hp, hp, 0, nx * hp, 0, (long long)nx * hp, 0);
     gy.xcd = 1;
     VK_TRY(launch_fwd(FWD_PLAIN, ny, ny, 1, F64, p->t1.p, p->t2.p, p->tw_y.p, gy, ny, 1.0, 1.0, nullptr, p->st));
-    // x forward: t2 [iy][ix][hp] -> the caller's layout [ix][iy][hz]: column
-    // (iy, iz) of the padded column space stores at iy*hz + iz, valid
-    // columns only.  Tiles run per XCD, so the lines that neighbouring tiles'
-    // unaligned rows share are written in one L2 (384^3 R2C 0.670 -> 0.609 ms).
-    Strided gx = make_strided(ny * hp, hp, hz, hp, (long long)nx * hp, hp, 0, hp, hz, (long long)ny * hz, 0);
+    // x forward: t2 [iy][ix][hp] -> the caller's layout [ix][iy][hz].  Its
+    // tiles run over the unpadded column space: column c = iy*hz + iz stores
+    // at offset c, so a tile of 8 columns stores whole aligned 128-B lines
+    // (tiles of the padded column space stored 128-B pieces at an offset of
+    // 16 B mod 32 into rows of hz = 193 entries, leaving partial 32-B sectors
+    // at both ends).  A tile may then span two rows of t2 on the input side,
+    // where partial lines cost less.  Tiles run per XCD, so neighbouring tiles'
+    // shared input lines are read in one L2.
+    Strided gx = make_strided(ny * hz, hz, hz, hz, (long long)nx * hp, hp, 0, hz, hz, (long long)ny * hz, 0);
     gx.xcd = 1;
     VK_TRY(launch_fwd(FWD_PLAIN, nx, nx, 1, F64, p->t2.p, spec, p->tw_x.p, gx, nx, 1.0, 1.0, nullptr, p->st));
     return 0;
