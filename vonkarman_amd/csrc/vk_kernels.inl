@@ -1356,13 +1356,50 @@ __device__ __forceinline__ void zw_twiddle(cpx<R> (&v)[N], cpx<R> w1) {
     }
 }
 
-// wa = W_384^b (lane b), wb = W_64^{j % 8}.  The caller synchronises before
-// its next LDS write.
-template <int D, typename R>
-__device__ __forceinline__ void zw_64to48(cpx<R> (&v6)[6], cpx<R> (&v8)[8], cpx<R> *lds, int lane, int j, cpx<R> wa,
-                                          cpx<R> wb) {
+// The twiddles of a lane, wa = W_384^b (lane b) and wb = W_64^{j % 8}: the
+// bases only, their powers formed in every transform (TC = false), or all
+// twelve powers formed once (TC = true: 10 complex registers more, 10 complex
+// products fewer per transform).
+template <typename R, bool TC>
+struct ZwTw;
+template <typename R>
+struct ZwTw<R, false> {
+    cpx<R> wa, wb;
+    __device__ __forceinline__ ZwTw(cpx<R> a, cpx<R> b) : wa(a), wb(b) {}
+    template <int D>
+    __device__ __forceinline__ void a6(cpx<R> (&v)[6]) const { zw_twiddle<D>(v, wa); }
+    template <int D>
+    __device__ __forceinline__ void b8(cpx<R> (&v)[8]) const { zw_twiddle<D>(v, wb); }
+};
+template <typename R>
+struct ZwTw<R, true> {
+    cpx<R> pa[5], pb[7];  // w^1..w^5 of wa, w^1..w^7 of wb
+    __device__ __forceinline__ ZwTw(cpx<R> a, cpx<R> b) {
+        pa[0] = a;
+        pb[0] = b;
+#pragma unroll
+        for (int m = 1; m < 5; ++m) pa[m] = cmul(pa[m - 1], a);
+#pragma unroll
+        for (int m = 1; m < 7; ++m) pb[m] = cmul(pb[m - 1], b);
+    }
+    template <int D, int N>
+    __device__ __forceinline__ static void ap(cpx<R> (&v)[N], const cpx<R> *p) {
+#pragma unroll
+        for (int m = 1; m < N; ++m) v[m] = D < 0 ? cmul(v[m], p[m - 1]) : cmulc(v[m], p[m - 1]);
+    }
+    template <int D>
+    __device__ __forceinline__ void a6(cpx<R> (&v)[6]) const { ap<D>(v, pa); }
+    template <int D>
+    __device__ __forceinline__ void b8(cpx<R> (&v)[8]) const { ap<D>(v, pb); }
+};
+
+// tw: the lane's twiddles (ZwTw).  The caller synchronises before its next
+// LDS write.
+template <int D, typename R, typename TWS>
+__device__ __forceinline__ void zw_64to48(cpx<R> (&v6)[6], cpx<R> (&v8)[8], cpx<R> *lds, int lane, int j,
+                                          const TWS &tw) {
     dft<6, D>(v6);
-    zw_twiddle<D>(v6, wa);
+    tw.template a6<D>(v6);
 #pragma unroll
     for (int c = 0; c < 6; ++c) lds[zw::x1(c, lane)] = v6[c];
     __syncthreads();
@@ -1371,7 +1408,7 @@ __device__ __forceinline__ void zw_64to48(cpx<R> (&v6)[6], cpx<R> (&v8)[8], cpx<
 #pragma unroll
         for (int b1 = 0; b1 < 8; ++b1) v8[b1] = lds[zw::x1(c, 8 * b1 + b0)];
         dft<8, D>(v8);
-        zw_twiddle<D>(v8, wb);
+        tw.template b8<D>(v8);
         __syncthreads();
         if (lane < 48) {
 #pragma unroll
@@ -1387,9 +1424,9 @@ __device__ __forceinline__ void zw_64to48(cpx<R> (&v6)[6], cpx<R> (&v8)[8], cpx<
     }
 }
 
-template <int D, typename R>
-__device__ __forceinline__ void zw_48to64(cpx<R> (&v8)[8], cpx<R> (&v6)[6], cpx<R> *lds, int lane, int j, cpx<R> wa,
-                                          cpx<R> wb) {
+template <int D, typename R, typename TWS>
+__device__ __forceinline__ void zw_48to64(cpx<R> (&v8)[8], cpx<R> (&v6)[6], cpx<R> *lds, int lane, int j,
+                                          const TWS &tw) {
     dft<8, D>(v8);
     if (lane < 48) {
         const int c = j % 6, d0 = j / 6;
@@ -1401,7 +1438,7 @@ __device__ __forceinline__ void zw_48to64(cpx<R> (&v8)[8], cpx<R> (&v6)[6], cpx<
         const int c = j >> 3, b0 = j & 7;
 #pragma unroll
         for (int d0 = 0; d0 < 8; ++d0) v8[d0] = lds[zw::x2(c, d0, b0)];
-        zw_twiddle<D>(v8, wb);
+        tw.template b8<D>(v8);
         dft<8, D>(v8);
         __syncthreads();
         if (lane < 48) {
@@ -1412,7 +1449,7 @@ __device__ __forceinline__ void zw_48to64(cpx<R> (&v8)[8], cpx<R> (&v6)[6], cpx<
     __syncthreads();
 #pragma unroll
     for (int c = 0; c < 6; ++c) v6[c] = lds[zw::x1(c, lane)];
-    zw_twiddle<D>(v6, wa);
+    tw.template a6<D>(v6);
     dft<6, D>(v6);
 }
 
@@ -1456,7 +1493,7 @@ __device__ __forceinline__ void zw_combine(cpx<TW> (&v)[ZwSpec<S64>::NS], const 
 // transform.  WPE: the waves-per-SIMD floor asked of the compiler.  LPB lines
 // per block (1 or 2): with two, the c2 of the first line waits in LDS and the
 // two c2 share one forward transform (9 transforms per two lines, not 10).
-template <typename TW, bool S64, bool PF, int WPE, int LPB>
+template <typename TW, bool S64, bool PF, int WPE, int LPB, bool TC = false>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void k_z_nl_w(
     const cpx<TW> *__restrict__ in, cpx<TW> *__restrict__ out, long long lines, int hp, long long fs_in,
     long long fs_out, TW s_in, TW s_out, const cpx<TW> *__restrict__ tw) {
@@ -1470,13 +1507,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     const int j = lane < 48 ? lane : lane - 48;
     const bool wr = S64 || lane < 48;  // lanes that own spectral slots
     const gptr<cpx<TW>> gtw = (gptr<cpx<TW>>)tw;
-    const cpx<TW> wa = ldg(gtw, lane), wb = ldg(gtw, 6 * (j & 7));
+    const ZwTw<TW, TC> tws(ldg(gtw, lane), ldg(gtw, 6 * (j & 7)));
     const cpx<TW> zero = {TW(0), TW(0)};
     const TW h = TW(0.5) * s_out;
     cpx<TW> vs[NS], vp[NP];
     auto forward = [&]() {
-        if constexpr (S64) zw_48to64<-1>(vp, vs, lds, lane, j, wa, wb);
-        else zw_64to48<-1>(vp, vs, lds, lane, j, wa, wb);
+        if constexpr (S64) zw_48to64<-1>(vp, vs, lds, lane, j, tws);
+        else zw_64to48<-1>(vp, vs, lds, lane, j, tws);
     };
     // spectral slots holding kept modes: k < 192 (S64) or k < 144
     constexpr int NK = 3;
@@ -1536,8 +1573,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
                 zw_after(vs);
                 issue(l, q + 1);
             }
-            if constexpr (S64) zw_64to48<+1>(vs, vp, lds, lane, j, wa, wb);
-            else zw_48to64<+1>(vs, vp, lds, lane, j, wa, wb);
+            if constexpr (S64) zw_64to48<+1>(vs, vp, lds, lane, j, tws);
+            else zw_48to64<+1>(vs, vp, lds, lane, j, tws);
 #pragma unroll
             for (int e = 0; e < NP; ++e) {
                 r[2 * q][e] = vp[e].re * s_in;
@@ -1718,7 +1755,7 @@ __global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) __attribute__((amdgpu_waves_p
 
 // The VJP z pass at L = 384 with one line per wavefront (the layouts of
 // k_z_nl_w): the same fields, transforms and output as k_z_vjp.
-template <typename TW, bool S64, int WPE>
+template <typename TW, bool S64, int WPE, bool TC = false>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void k_z_vjp_w(
     const cpx<TW> *__restrict__ in, cpx<TW> *__restrict__ out, long long lines, int hp, long long fs_in,
     long long fs_out, TW s_u, TW s_o, const cpx<TW> *__restrict__ tw) {
@@ -1731,7 +1768,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     const bool wr = S64 || lane < 48;  // lanes that own spectral slots
     const long long base = (long long)blockIdx.x * hp;  // one line per block, grid = lines
     const gptr<cpx<TW>> gtw = (gptr<cpx<TW>>)tw;
-    const cpx<TW> wa = ldg(gtw, lane), wb = ldg(gtw, 6 * (j & 7));
+    const ZwTw<TW, TC> tws(ldg(gtw, lane), ldg(gtw, 6 * (j & 7)));
     const cpx<TW> zero = {TW(0), TW(0)};
     const TW one = TW(1), two = TW(2), half = TW(0.5);
     cpx<TW> vs[NS], vp[NP];
@@ -1751,16 +1788,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
         }
         // position 0 is slot 0 of lane 0 in both layouts
         if (lane == 0) vs[0] = {vs[0].re * dca, vs[0].im * dcb};
-        if constexpr (S64) zw_64to48<+1>(vs, vp, lds, lane, j, wa, wb);
-        else zw_48to64<+1>(vs, vp, lds, lane, j, wa, wb);
+        if constexpr (S64) zw_64to48<+1>(vs, vp, lds, lane, j, tws);
+        else zw_48to64<+1>(vs, vp, lds, lane, j, tws);
     };
     // forward DFT of vp split into the truncated half spectra of its real and
     // imaginary parts, times c_k s (c_0 = 1, c_k = 2): the transpose of the
     // unnormalised C2R.  Lines are written whole, entries [HOUT, hp) as zeros.
     constexpr int NK = 3, PLO = S64 ? 256 : 240;
     auto forward_split = [&](cpx<TW> *oa, cpx<TW> *ob) {
-        if constexpr (S64) zw_48to64<-1>(vp, vs, lds, lane, j, wa, wb);
-        else zw_64to48<-1>(vp, vs, lds, lane, j, wa, wb);
+        if constexpr (S64) zw_48to64<-1>(vp, vs, lds, lane, j, tws);
+        else zw_64to48<-1>(vp, vs, lds, lane, j, tws);
         __syncthreads();
         if (wr) {
 #pragma unroll
@@ -2172,7 +2209,8 @@ int launch_z_nl_L(Prec wp, const void *in, void *out, long long lines, int hp, l
         //   measured the same.
         //   FP32: spectral side in layout 64 with prefetch, one line per block
         //   (92 VGPRs, five waves): 0.326 ms; two lines per block 0.331,
-        //   without prefetch 0.347, layout 48 0.356.
+        //   without prefetch 0.347, layout 48 0.356.  The twiddle powers formed
+        //   once per wave (99 VGPRs, four waves): 0.326 -> 0.321 ms.
         if (wp == F64) {
             if (lines % 2 == 0)
                 hipLaunchKernelGGL((k_z_nl_w<double, false, false, 3, 2>), dim3((unsigned)(lines / 2)), dim3(64), 0,
@@ -2183,7 +2221,7 @@ int launch_z_nl_L(Prec wp, const void *in, void *out, long long lines, int hp, l
                                    reinterpret_cast<const cpx<double> *>(in), reinterpret_cast<cpx<double> *>(out),
                                    lines, hp, fs_in, fs_out, s_in, s_out, reinterpret_cast<const cpx<double> *>(tw));
         } else {
-            hipLaunchKernelGGL((k_z_nl_w<float, true, true, 4, 1>), dim3((unsigned)lines), dim3(64), 0, st,
+            hipLaunchKernelGGL((k_z_nl_w<float, true, true, 4, 1, true>), dim3((unsigned)lines), dim3(64), 0, st,
                                reinterpret_cast<const cpx<float> *>(in), reinterpret_cast<cpx<float> *>(out), lines,
                                hp, fs_in, fs_out, float(s_in), float(s_out), reinterpret_cast<const cpx<float> *>(tw));
         }
@@ -2215,13 +2253,15 @@ int launch_z_vjp_L(Prec wp, const void *in, void *out, long long lines, int hp, 
         // with the spectral side in layout 48 at two waves (196 VGPRs; a
         // 3-wave floor spills 156 B, 108 B with W1, W2 held in LDS, and ran
         // 18.7 ms); mixed 10.94 -> 10.09 ms with the spectral side in layout
-        // 64 (112 VGPRs, four waves; layout 48 10.33 ms)
+        // 64 (112 VGPRs, four waves; layout 48 10.33 ms).  The twiddle powers
+        // formed once per wave keep those occupancies (206 / 119 VGPRs):
+        // reverse step 16.84 -> 16.71 ms FP64, 9.98 -> 9.91 ms mixed.
         if (wp == F64)
-            hipLaunchKernelGGL((k_z_vjp_w<double, false, 2>), dim3((unsigned)lines), dim3(64), 0, st,
+            hipLaunchKernelGGL((k_z_vjp_w<double, false, 2, true>), dim3((unsigned)lines), dim3(64), 0, st,
                                reinterpret_cast<const cpx<double> *>(in), reinterpret_cast<cpx<double> *>(out), lines,
                                hp, fs_in, fs_out, s_u, s_o, reinterpret_cast<const cpx<double> *>(tw));
         else
-            hipLaunchKernelGGL((k_z_vjp_w<float, true, 4>), dim3((unsigned)lines), dim3(64), 0, st,
+            hipLaunchKernelGGL((k_z_vjp_w<float, true, 4, true>), dim3((unsigned)lines), dim3(64), 0, st,
                                reinterpret_cast<const cpx<float> *>(in), reinterpret_cast<cpx<float> *>(out), lines,
                                hp, fs_in, fs_out, float(s_u), float(s_o), reinterpret_cast<const cpx<float> *>(tw));
         return hipGetLastError() == hipSuccess ? 0 : 3;
