@@ -200,14 +200,42 @@ VK_HD void dft_ct(cpx<R> *v) {
     }
 }
 
+// composite RR = A*B with gcd(A, B) = 1 by the prime-factor (Good-Thomas)
+// mapping, which needs no twiddles: input n = (B n1 + A n2) mod RR, output k
+// with k = k1 (mod A) and k = k2 (mod B), i.e. k = (B (B^-1 mod A) k1 +
+// A (A^-1 mod B) k2) mod RR, and w_RR^{nk} = w_A^{n1 k1} w_B^{n2 k2}.
+constexpr int inv_mod(int a, int m) {
+    for (int x = 1; x < m; ++x)
+        if ((a * x) % m == 1) return x;
+    return 1;
+}
+template <int A, int B, int D, typename R>
+VK_HD void dft_pfa(cpx<R> *v) {
+    constexpr int RR = A * B;
+    constexpr int CA = B * inv_mod(B % A, A), CB = A * inv_mod(A % B, B);
+    cpx<R> y[A][B];
+    for (int n1 = 0; n1 < A; ++n1) {
+        cpx<R> t[B];
+        for (int n2 = 0; n2 < B; ++n2) t[n2] = v[(B * n1 + A * n2) % RR];
+        dft<B, D>(t);
+        for (int k2 = 0; k2 < B; ++k2) y[n1][k2] = t[k2];
+    }
+    for (int k2 = 0; k2 < B; ++k2) {
+        cpx<R> u[A];
+        for (int n1 = 0; n1 < A; ++n1) u[n1] = y[n1][k2];
+        dft<A, D>(u);
+        for (int k1 = 0; k1 < A; ++k1) v[(CA * k1 + CB * k2) % RR] = u[k1];
+    }
+}
+
 template <int RAD, int D, typename R>
 VK_HD void dft(cpx<R> *v) {
     if constexpr (RAD == 2) dft2<D>(v);
     else if constexpr (RAD == 3) dft3<D>(v);
     else if constexpr (RAD == 4) dft4<D>(v);
-    else if constexpr (RAD == 6) dft_ct<2, 3, D>(v);
+    else if constexpr (RAD == 6) dft_pfa<2, 3, D>(v);
     else if constexpr (RAD == 8) dft8<D>(v);
-    else if constexpr (RAD == 12) dft_ct<4, 3, D>(v);
+    else if constexpr (RAD == 12) dft_pfa<4, 3, D>(v);
     else if constexpr (RAD == 16) dft_ct<4, 4, D>(v);
     else static_assert(RAD == 2, "unsupported radix");
 }

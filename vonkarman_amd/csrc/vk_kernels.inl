@@ -1509,7 +1509,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     const gptr<cpx<TW>> gtw = (gptr<cpx<TW>>)tw;
     const ZwTw<TW, TC> tws(ldg(gtw, lane), ldg(gtw, 6 * (j & 7)));
     const cpx<TW> zero = {TW(0), TW(0)};
-    const TW h = TW(0.5) * s_out;
+    // The input scale s_in is applied at the output: (s u) x (s w) = s^2 (u x w)
+    // and the forward transform is linear, so the physical fields stay
+    // unscaled and the kept modes take s_out s_in^2 (exactly the same values
+    // when s_in is a power of two, as 1/N^3 is for the power-of-two grids).
+    const TW so = s_out * (s_in * s_in);
+    const TW h = TW(0.5) * so;
     cpx<TW> vs[NS], vp[NP];
     auto forward = [&]() {
         if constexpr (S64) zw_48to64<-1>(vp, vs, lds, lane, j, tws);
@@ -1577,8 +1582,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
             else zw_48to64<+1>(vs, vp, lds, lane, j, tws);
 #pragma unroll
             for (int e = 0; e < NP; ++e) {
-                r[2 * q][e] = vp[e].re * s_in;
-                r[2 * q + 1][e] = vp[e].im * s_in;
+                r[2 * q][e] = vp[e].re;
+                r[2 * q + 1][e] = vp[e].im;
             }
             __syncthreads();
         }
@@ -1616,7 +1621,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
 #pragma unroll
             for (int s = 0; s < NK; ++s) {
                 const int k = SP::pos(lane, j, s);
-                if (wr && (s < NK - 1 || k < hp)) o2[k] = k < HOUT ? cscale(vs[s], s_out) : zero;
+                if (wr && (s < NK - 1 || k < hp)) o2[k] = k < HOUT ? cscale(vs[s], so) : zero;
             }
         }
     }
