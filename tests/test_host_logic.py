@@ -58,6 +58,19 @@ def test_fft_core_index_arithmetic_on_host(tmp_path):
     assert out.stdout.strip().endswith("OK")
 
 
+def test_one_line_z_transform_layouts_on_host(tmp_path):
+    """The 384-point transforms of the one-line-per-wavefront z passes, run lane
+    by lane on the host (tests/host/zw_sim.cpp): both layouts and directions
+    against a direct DFT, every LDS exchange entry written once before it is
+    read, and bank-conflict-free exchange accesses."""
+    exe = tmp_path / "zwsim"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "vonkarman_amd", "csrc"),
+                    os.path.join(ROOT, "tests", "host", "zw_sim.cpp"), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert out.returncode == 0, out.stdout[-2000:]
+    assert out.stdout.strip().endswith("OK")
+
+
 def test_taylor_green_ic_bit_exact_with_oracle():
     """vk_ic_taylor_green (host code of the library) evaluates the reference's
     formula exactly as the oracle does (taylor_green.rs:10-38)."""

@@ -621,6 +621,27 @@ __device__ __forceinline__ void twiddles_store(cpx<R> *ltw, const TwStage<W, NT,
 #endif
 
 // ---------------------------------------------------------------------------
+// Index maps of the one-line-per-wavefront 384-point transforms (k_z_nl_w,
+// k_z_vjp_w): 384 = 6 * 8 * 8 over a 64-lane wave.
+//   layout 64: lane b holds positions 64 a + b, a = 0..5;
+//   layout 48: lane j = c + 6 d0 (j < 48) holds positions j + 48 d1, d1 = 0..7.
+// ---------------------------------------------------------------------------
+namespace zw {
+constexpr int L = 384, HIN = L / 3 + 1;
+// exchange next to the radix-6 pass: (c, b) -> c * 72 + b.  Lanes 8 c + b0
+// read b = 8 b1 + b0: 16-B elements of 8 adjacent lanes, and 8-B elements of
+// 16 adjacent lanes (c * 72 = 8 mod 16), fall in distinct banks.
+VK_HD int x1(int c, int b) { return c * 72 + b; }
+// exchange between the radix-8 passes: (c, d0, b0) -> b0 + 9 c + 54 d0.  The
+// lanes 8 c + b0 (b0 across adjacent lanes) and the lanes c + 6 d0 (entry
+// = lane mod 8) both touch distinct banks with 16-B elements.
+VK_HD int x2(int c, int d0, int b0) { return b0 + 9 * c + 54 * d0; }
+constexpr int NLDS = 432;
+// position of element d1 of lane j (j < 48) in layout 48
+VK_HD int pos48(int j, int d1) { return j + 48 * d1; }
+}  // namespace zw
+
+// ---------------------------------------------------------------------------
 // dealiasing index maps (the reference's zero_pad_spectral / truncate_spectral
 // maps, vonkarman-fft/src/dealiased.rs:127-128,150-155, applied per axis)
 // ---------------------------------------------------------------------------

@@ -1327,20 +1327,7 @@ __global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) __attribute__((amdgpu_waves_p
 // share a line, so the pass runs at more waves per SIMD; each transform makes
 // two LDS exchanges instead of three.
 // ===========================================================================
-namespace zw {
-constexpr int L = 384, HIN = L / 3 + 1;
-// exchange next to the radix-6 pass: (c, b) -> c * 72 + b.  Lanes 8 c + b0
-// read b = 8 b1 + b0: 16-B elements of 8 adjacent lanes, and 8-B elements of
-// 16 adjacent lanes (c * 72 = 8 mod 16), fall in distinct banks.
-__device__ __forceinline__ int x1(int c, int b) { return c * 72 + b; }
-// exchange between the radix-8 passes: (c, d0, b0) -> b0 + 9 c + 54 d0.  The
-// lanes 8 c + b0 (b0 across adjacent lanes) and the lanes c + 6 d0 (entry
-// = lane mod 8) both touch distinct banks.
-__device__ __forceinline__ int x2(int c, int d0, int b0) { return b0 + 9 * c + 54 * d0; }
-constexpr int NLDS = 432;
-// position of element d1 of lane j (j < 48) in layout 48
-__device__ __forceinline__ int pos48(int j, int d1) { return j + 48 * d1; }
-}  // namespace zw
+// index maps of the two layouts and the exchanges: zw:: in vk_fft.hpp
 
 // v[m] *= w^m (D < 0) or conj(w)^m (D > 0), m = 1..N-1.  The powers are
 // formed anew in every transform: w1 passes through an empty asm statement,
