@@ -1320,12 +1320,14 @@ __global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) __attribute__((amdgpu_waves_p
 //   over b0,
 // and zw_48to64 the same passes in reverse order, each twiddle applied before
 // its butterfly (the DFT matrix is symmetric, so both compute the DFT of
-// direction D, from one layout into the other).  The spectral side uses
-// layout 48 and the physical side layout 64: the six physical fields of the
-// cross product take 6 elements per lane.  A lane holds at most four fields
-// plus one transform at once, against 12 elements per field when 32 lanes
-// share a line, so the pass runs at more waves per SIMD; each transform makes
-// two LDS exchanges instead of three.
+// direction D, from one layout into the other).  Which side takes which
+// layout is a template choice (S64): FP64 puts the spectral side in layout 48
+// and the physical side in layout 64, so the six physical fields of the cross
+// product take 6 elements per lane; FP32 puts the spectral side in layout 64
+// (every lane loads and stores).  A lane holds at most four fields plus one
+// transform at once, against 12 elements per field when 32 lanes share a
+// line, so the pass runs at more waves per SIMD; each transform makes two LDS
+// exchanges instead of three.
 // ===========================================================================
 // index maps of the two layouts and the exchanges: zw:: in vk_fft.hpp
 
