@@ -176,7 +176,8 @@ int vk_solver_cfl_dt(vk_solver *s, double *v);        /* solver.rs:569-577 */
 /* ScalarDiagnostics::from_domain (scalar.rs:22-46) in one device pass. */
 int vk_solver_diagnostics(vk_solver *s, vk_diag *d);
 /* energy_spectrum (vonkarman-diag/src/spectrum.rs:13-60): writes
- * min(cap, n_shells) shells and stores n_shells = ceil(sqrt(3) n/2) + 1. */
+ * min(cap, n_shells) shells and stores n_shells = ceil(sqrt(3) n/2) + 1.
+ * With out NULL or cap <= 0 it only stores n_shells (no device work). */
 int vk_solver_energy_spectrum(vk_solver *s, double *out, int cap, int *n_shells);
 /* compute_nonlinear (vonkarman-periodic/src/nonlinear.rs:15-39) on a host
  * state: nhat = P trunc FFT(u x curl u).  Does not touch the solver state. */
