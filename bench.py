@@ -497,16 +497,23 @@ def pmc_child(n: int, precision: str = "f64") -> None:
     s.steps(1)
 
 
+REQ_COUNTERS = ("TCC_EA0_RDREQ_32B_sum", "TCC_EA0_RDREQ_64B_sum", "TCC_EA0_RDREQ_128B_sum")
+
+
 def measure_traffic(n: int, pass_name: str, timeout: float = 150.0) -> dict:
-    """HBM bytes per launch of one pass from two rocprofv3 --pmc runs
-    (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch).  FETCH_SIZE is scaled by
-    the ratio of known bytes to the counter's reading for the state repack
-    kernel of the same run (a plain copy that reads 3 n^2 rows of n/2+1
-    complex doubles), WRITE_SIZE by the same ratio for its write (3 n^2 rows
-    of the padded row pitch).  On failure returns {"error": reason}."""
+    """HBM bytes per launch of one pass from three rocprofv3 --pmc runs.
+
+    * Reads: the L2's read requests to memory by size (TCC_EA0_RDREQ_32B,
+      _64B and _128B, summed over channels) times their sizes.  No
+      calibration: for the state repack kernel of the same run (a copy that
+      reads 3 n^2 rows of n/2+1 complex doubles) they give its known bytes
+      within the partial lines of its unaligned rows.
+    * Writes: WRITE_SIZE (KiB per dispatch).
+    * Cross-check: FETCH_SIZE scaled by the ratio of the repack kernel's known
+      bytes to its reading (FETCH_SIZE counts 128-B requests as 64 B).
+    On failure returns {"error": reason}."""
     import re
     import shutil
-    import sqlite3
     import subprocess
     import tempfile
 
@@ -520,49 +527,51 @@ def measure_traffic(n: int, pass_name: str, timeout: float = 150.0) -> dict:
         want = "k_z_nl_w<double, "
     known_read_kib = 3 * n * n * H * 16 / 1024.0
     known_write_kib = 3 * n * n * ((H + 7) // 8 * 8) * 16 / 1024.0
-    vals = {}
-    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+    vals = {}  # counter set -> (repack value, kernel value)
+    for counters in (("FETCH_SIZE",), ("WRITE_SIZE",), REQ_COUNTERS):
         d = tempfile.mkdtemp(prefix="vk_pmc_", dir=os.environ.get("TMPDIR", "/tmp"))
-        cmd = [prof, "--pmc", counter, "-d", d, "-o", "run", "--", sys.executable,
+        cmd = [prof, "--pmc", *counters, "-d", d, "-o", "run", "--output-format", "csv", "--", sys.executable,
                os.path.abspath(__file__), "--pmc-child", "--n", str(n)]
+        tag = counters[0] if len(counters) == 1 else "RDREQ by size"
         try:
             cp = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, timeout=timeout, check=False)
         except subprocess.TimeoutExpired:
-            return {"error": f"{counter} run timed out after {timeout} s"}
-        dbs = [os.path.join(r, f) for r, _, fs in os.walk(d) for f in fs if f.endswith("_results.db")]
+            return {"error": f"{tag} run timed out after {timeout} s"}
         csvs = [os.path.join(r, f) for r, _, fs in os.walk(d) for f in fs if f.endswith("counter_collection.csv")]
-        rows = []
-        if dbs:
-            con = sqlite3.connect(dbs[0])
-            tables = [t for (t,) in con.execute("select name from sqlite_master where type in ('table','view')")]
-            tab = next((t for t in tables if t == "counters_collection"), None)
-            if tab is None:
-                tail = cp.stdout.decode(errors="replace")[-300:]
-                return {"error": f"no counters_collection table in {os.path.basename(dbs[0])} "
-                                 f"(tables: {tables[:12]}); rc={cp.returncode}; tail: {tail}"}
-            rows = list(con.execute(f"select kernel_name, value from {tab} order by dispatch_id"))
-        elif csvs:
-            import csv
-
-            with open(csvs[0]) as fh:
-                rows = [(r["Kernel_Name"], float(r["Counter_Value"])) for r in csv.DictReader(fh)]
-        else:
+        if not csvs:
             tail = cp.stdout.decode(errors="replace")[-400:]
-            return {"error": f"{counter}: no results (rc={cp.returncode}): {tail}"}
-        short = [(re.sub(r"\(.*", "", k).replace("void ", "").replace("vk::", ""), v) for k, v in rows]
+            shutil.rmtree(d, ignore_errors=True)
+            return {"error": f"{tag}: no results (rc={cp.returncode}): {tail}"}
+        import csv
+
+        with open(csvs[0]) as fh:
+            rows = [(r["Kernel_Name"], r["Dispatch_Id"], r["Counter_Name"], float(r["Counter_Value"]))
+                    for r in csv.DictReader(fh)]
+        shutil.rmtree(d, ignore_errors=True)
+        # bytes (KiB for the single counters) per dispatch, then per kernel
+        per = {}
+        for k, disp, cn, v in rows:
+            key = (re.sub(r"\(.*", "", k).replace("void ", "").replace("vk::", ""), disp)
+            w = {"TCC_EA0_RDREQ_32B_sum": 32.0, "TCC_EA0_RDREQ_64B_sum": 64.0,
+                 "TCC_EA0_RDREQ_128B_sum": 128.0}.get(cn, 1.0)
+            per[key] = per.get(key, 0.0) + v * w
+        short = [(k, v) for (k, _), v in per.items()]
         rep = [v for k, v in short if k.startswith("k_repack")]
         ker = [v for k, v in short if k.startswith(want)]
-        shutil.rmtree(d, ignore_errors=True)
         if not rep or not ker:
             names = sorted({k for k, _ in short})[:10]
-            return {"error": f"{counter}: kernel rows missing (repack {len(rep)}, {want} {len(ker)}); seen {names}"}
-        vals[counter] = (rep[0], sum(ker) / len(ker))
+            return {"error": f"{tag}: kernel rows missing (repack {len(rep)}, {want} {len(ker)}); seen {names}"}
+        vals[tag] = (rep[0], sum(ker) / len(ker))
     fetch_scale = known_read_kib / vals["FETCH_SIZE"][0]
     write_scale = known_write_kib / vals["WRITE_SIZE"][0]
-    read_b = vals["FETCH_SIZE"][1] * fetch_scale * 1024.0
-    write_b = vals["WRITE_SIZE"][1] * write_scale * 1024.0
-    return {"bytes": read_b + write_b, "read_bytes": read_b, "write_bytes": write_b,
-            "fetch_scale": fetch_scale, "write_scale": write_scale, "kernel": want}
+    read_b = vals["RDREQ by size"][1]
+    write_b = vals["WRITE_SIZE"][1] * 1024.0
+    return {"bytes": read_b + write_b, "read_bytes": read_b, "write_bytes": write_b, "kernel": want,
+            "read_source": "TCC_EA0_RDREQ_{32B,64B,128B}_sum x size (no calibration)",
+            "repack_read_over_known": vals["RDREQ by size"][0] / (known_read_kib * 1024.0),
+            "repack_write_over_known": 1.0 / write_scale,
+            "read_bytes_fetch_size_calibrated": vals["FETCH_SIZE"][1] * fetch_scale * 1024.0,
+            "fetch_scale": fetch_scale}
 
 
 def _host_cpu() -> dict:
