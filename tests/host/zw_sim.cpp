@@ -8,8 +8,8 @@
 //   * each exchange writes every entry it reads, exactly once, inside the
 //     LDS buffer (zw::NLDS entries);
 //   * the lanes of each group that the LDS serves together touch distinct
-//     banks: 8 lanes of 16-B elements (FP64) and, for x1, 16 lanes of 8-B
-//     elements (FP32).
+//     banks: 8 lanes of 16-B elements (FP64) and 16 lanes of 8-B elements
+//     (FP32), each with its element size's exchange layout.
 // Exit status 1 on any mismatch.
 #include <cmath>
 #include <complex>
@@ -75,8 +75,9 @@ static void banks(const std::vector<int> &addr, int lanes_per_group, int elems_p
     }
 }
 
-// layout 64 -> layout 48 (zw_64to48): in[lane][a] = x[64 a + lane]
-template <int D>
+// layout 64 -> layout 48 (zw_64to48): in[lane][a] = x[64 a + lane]; EB: the
+// element bytes whose exchange layout is used (16 FP64, 8 FP32)
+template <int D, int EB>
 static void t64to48(const C (&in)[64][6], C (&out)[48][8]) {
     C v6[64][6], v8[64][8];
     Exchange e1, e2;
@@ -106,28 +107,33 @@ static void t64to48(const C (&in)[64][6], C (&out)[48][8]) {
     }
     for (int lane = 0; lane < 48; ++lane) {
         const int c = lane >> 3, b0 = lane & 7;
-        for (int d0 = 0; d0 < 8; ++d0) e2.write(zw::x2(c, d0, b0), v8[lane][d0], "x2 (64->48)");
+        for (int d0 = 0; d0 < 8; ++d0) e2.write(zw::x2<EB>(c, d0, b0), v8[lane][d0], "x2 (64->48)");
     }
     for (int lane = 0; lane < 64; ++lane) {
         const int j = jlane(lane), c = j % 6, d0 = j / 6;
-        for (int b0 = 0; b0 < 8; ++b0) v8[lane][b0] = e2.read(zw::x2(c, d0, b0), "x2 (64->48)");
+        for (int b0 = 0; b0 < 8; ++b0) v8[lane][b0] = e2.read(zw::x2<EB>(c, d0, b0), "x2 (64->48)");
         dft<8, D>(v8[lane]);
     }
     for (int k = 0; k < 8; ++k) {
         std::vector<int> w, r;
         for (int lane = 0; lane < 48; ++lane) {
-            w.push_back(zw::x2(lane >> 3, k, lane & 7));
-            r.push_back(zw::x2(lane % 6, lane / 6, k));
+            w.push_back(zw::x2<EB>(lane >> 3, k, lane & 7));
+            r.push_back(zw::x2<EB>(lane % 6, lane / 6, k));
         }
-        banks(w, 8, 8, "x2 write, 16-B");
-        banks(r, 8, 8, "x2 read, 16-B");
+        if (EB == 16) {
+            banks(w, 8, 8, "x2 write, 16-B");
+            banks(r, 8, 8, "x2 read, 16-B");
+        } else {
+            banks(w, 16, 16, "x2 write, 8-B");
+            banks(r, 16, 16, "x2 read, 8-B");
+        }
     }
     for (int j = 0; j < 48; ++j)
         for (int d1 = 0; d1 < 8; ++d1) out[j][d1] = v8[j][d1];
 }
 
 // layout 48 -> layout 64 (zw_48to64): in[j][d1] = x[j + 48 d1]
-template <int D>
+template <int D, int EB>
 static void t48to64(const C (&in)[48][8], C (&out)[64][6]) {
     C v8[64][8], v6[64][6];
     Exchange e1, e2;
@@ -138,11 +144,11 @@ static void t48to64(const C (&in)[48][8], C (&out)[64][6]) {
     }
     for (int lane = 0; lane < 48; ++lane) {
         const int c = lane % 6, d0 = lane / 6;
-        for (int b0 = 0; b0 < 8; ++b0) e2.write(zw::x2(c, d0, b0), v8[lane][b0], "x2 (48->64)");
+        for (int b0 = 0; b0 < 8; ++b0) e2.write(zw::x2<EB>(c, d0, b0), v8[lane][b0], "x2 (48->64)");
     }
     for (int lane = 0; lane < 64; ++lane) {
         const int j = jlane(lane), c = j >> 3, b0 = j & 7;
-        for (int d0 = 0; d0 < 8; ++d0) v8[lane][d0] = e2.read(zw::x2(c, d0, b0), "x2 (48->64)");
+        for (int d0 = 0; d0 < 8; ++d0) v8[lane][d0] = e2.read(zw::x2<EB>(c, d0, b0), "x2 (48->64)");
         twiddle<D, 8>(v8[lane], tw384(6 * b0));
         dft<8, D>(v8[lane]);
     }
@@ -171,7 +177,7 @@ static std::vector<std::complex<long double>> direct(const std::vector<C> &x, in
     return X;
 }
 
-template <int D>
+template <int D, int EB>
 static void run(const std::vector<C> &x) {
     const auto X = direct(x, D);
     long double nrm = 0;
@@ -180,26 +186,26 @@ static void run(const std::vector<C> &x) {
         C in[64][6], out[48][8];
         for (int b = 0; b < 64; ++b)
             for (int a = 0; a < 6; ++a) in[b][a] = x[64 * a + b];
-        t64to48<D>(in, out);
+        t64to48<D, EB>(in, out);
         long double err = 0;
         for (int j = 0; j < 48; ++j)
             for (int d1 = 0; d1 < 8; ++d1) {
                 const int k = zw::pos48(j, d1);
                 err = std::fmax(err, std::abs(X[k] - std::complex<long double>(out[j][d1].re, out[j][d1].im)));
             }
-        std::printf("64->48 D=%+d: max rel err %.3Le\n", D, err / nrm);
+        std::printf("64->48 D=%+d EB=%d: max rel err %.3Le\n", D, EB, err / nrm);
         if (!(err / nrm < 1e-14)) ++fails;
     }
     {  // 48 -> 64
         C in[48][8], out[64][6];
         for (int j = 0; j < 48; ++j)
             for (int d1 = 0; d1 < 8; ++d1) in[j][d1] = x[zw::pos48(j, d1)];
-        t48to64<D>(in, out);
+        t48to64<D, EB>(in, out);
         long double err = 0;
         for (int b = 0; b < 64; ++b)
             for (int a = 0; a < 6; ++a)
                 err = std::fmax(err, std::abs(X[64 * a + b] - std::complex<long double>(out[b][a].re, out[b][a].im)));
-        std::printf("48->64 D=%+d: max rel err %.3Le\n", D, err / nrm);
+        std::printf("48->64 D=%+d EB=%d: max rel err %.3Le\n", D, EB, err / nrm);
         if (!(err / nrm < 1e-14)) ++fails;
     }
 }
@@ -216,8 +222,10 @@ int main() {
         }
     std::vector<C> x(384);
     for (int n = 0; n < 384; ++n) x[n] = {std::sin(0.37 * n + 0.1) + 0.01 * n, std::cos(1.3 * n * n + 0.2)};
-    run<+1>(x);
-    run<-1>(x);
+    run<+1, 16>(x);
+    run<-1, 16>(x);
+    run<+1, 8>(x);
+    run<-1, 8>(x);
     std::printf(fails ? "FAILED\n" : "OK\n");
     return fails ? 1 : 0;
 }

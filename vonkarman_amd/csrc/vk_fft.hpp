@@ -632,10 +632,18 @@ constexpr int L = 384, HIN = L / 3 + 1;
 // read b = 8 b1 + b0: 16-B elements of 8 adjacent lanes, and 8-B elements of
 // 16 adjacent lanes (c * 72 = 8 mod 16), fall in distinct banks.
 VK_HD int x1(int c, int b) { return c * 72 + b; }
-// exchange between the radix-8 passes: (c, d0, b0) -> b0 + 9 c + 54 d0.  The
-// lanes 8 c + b0 (b0 across adjacent lanes) and the lanes c + 6 d0 (entry
-// = lane mod 8) both touch distinct banks with 16-B elements.
-VK_HD int x2(int c, int d0, int b0) { return b0 + 9 * c + 54 * d0; }
+// exchange between the radix-8 passes, written by the lanes 8 c + b0 (b0
+// across adjacent lanes) and read by the lanes c + 6 d0.  Both touch distinct
+// banks: with 16-B elements (8 lanes at a time) as (c, d0, b0) ->
+// b0 + 9 c + 54 d0; with 8-B elements (16 lanes at a time) as
+// 8 c + c / 2 + 51 d0 + b0 (found by exhaustive search over layouts affine in
+// d0 and b0, which keep one address register per exchange; none serves both
+// element sizes).
+template <int EB>
+VK_HD int x2(int c, int d0, int b0) {
+    if constexpr (EB == 8) return 8 * c + (c >> 1) + 51 * d0 + b0;
+    else return b0 + 9 * c + 54 * d0;
+}
 constexpr int NLDS = 432;
 // position of element d1 of lane j (j < 48) in layout 48
 VK_HD int pos48(int j, int d1) { return j + 48 * d1; }

@@ -1401,14 +1401,14 @@ __device__ __forceinline__ void zw_64to48(cpx<R> (&v6)[6], cpx<R> (&v8)[8], cpx<
         __syncthreads();
         if (lane < 48) {
 #pragma unroll
-            for (int d0 = 0; d0 < 8; ++d0) lds[zw::x2(c, d0, b0)] = v8[d0];
+            for (int d0 = 0; d0 < 8; ++d0) lds[zw::x2<sizeof(cpx<R>)>(c, d0, b0)] = v8[d0];
         }
     }
     __syncthreads();
     {
         const int c = j % 6, d0 = j / 6;
 #pragma unroll
-        for (int b0 = 0; b0 < 8; ++b0) v8[b0] = lds[zw::x2(c, d0, b0)];
+        for (int b0 = 0; b0 < 8; ++b0) v8[b0] = lds[zw::x2<sizeof(cpx<R>)>(c, d0, b0)];
         dft<8, D>(v8);
     }
 }
@@ -1420,13 +1420,13 @@ __device__ __forceinline__ void zw_48to64(cpx<R> (&v8)[8], cpx<R> (&v6)[6], cpx<
     if (lane < 48) {
         const int c = j % 6, d0 = j / 6;
 #pragma unroll
-        for (int b0 = 0; b0 < 8; ++b0) lds[zw::x2(c, d0, b0)] = v8[b0];
+        for (int b0 = 0; b0 < 8; ++b0) lds[zw::x2<sizeof(cpx<R>)>(c, d0, b0)] = v8[b0];
     }
     __syncthreads();
     {
         const int c = j >> 3, b0 = j & 7;
 #pragma unroll
-        for (int d0 = 0; d0 < 8; ++d0) v8[d0] = lds[zw::x2(c, d0, b0)];
+        for (int d0 = 0; d0 < 8; ++d0) v8[d0] = lds[zw::x2<sizeof(cpx<R>)>(c, d0, b0)];
         tw.template b8<D>(v8);
         dft<8, D>(v8);
         __syncthreads();
