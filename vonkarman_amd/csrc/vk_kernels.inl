@@ -1749,7 +1749,7 @@ __global__ __launch_bounds__((ZNLGeom<L, TW>::NT)) __attribute__((amdgpu_waves_p
 
 // The VJP z pass at L = 384 with one line per wavefront (the layouts of
 // k_z_nl_w): the same fields, transforms and output as k_z_vjp.
-template <typename TW, bool S64, int WPE, bool TC = false>
+template <typename TW, bool S64, int WPE, bool TC = false, bool PF = false>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void k_z_vjp_w(
     const cpx<TW> *__restrict__ in, cpx<TW> *__restrict__ out, long long lines, int hp, long long fs_in,
     long long fs_out, TW s_u, TW s_o, const cpx<TW> *__restrict__ tw) {
@@ -1770,15 +1770,24 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     // inverse pair transform of fields (fa, fb) (fb < 0: none) with per-half
     // DC factors: the z transpose of an R2C counts its k=0 term once where the
     // C2R extension would count it twice
-    auto inverse = [&](int fa, int fb, TW dca, TW dcb) {
-        ZRaw<TW> raw[NS];
+    // The pairs in order: (6, 7), (8, none), (4, 5), (3, 0), (1, 2).  With PF
+    // each next pair's loads are issued once the current pair is combined, so
+    // they are in flight during its transform (and the forward transforms
+    // between the inverse ones).
+    ZRaw<TW> raw[NS];
+    auto issue = [&](int fa, int fb) {
         const gptr<cpx<TW>> pa = opaque(in) + fa * fs_in;
-        if (fb >= 0) {
-            zw_issue<S64, TW, true>(raw, pa, opaque(in) + fb * fs_in, base, lane, j);
-            zw_combine<S64, TW, true>(vs, raw, lane, j);
-        } else {
-            zw_issue<S64, TW, false>(raw, pa, pa, base, lane, j);
-            zw_combine<S64, TW, false>(vs, raw, lane, j);
+        if (fb >= 0) zw_issue<S64, TW, true>(raw, pa, opaque(in) + fb * fs_in, base, lane, j);
+        else zw_issue<S64, TW, false>(raw, pa, pa, base, lane, j);
+    };
+    if (PF) issue(6, 7);
+    auto inverse = [&](int fa, int fb, TW dca, TW dcb, int na = -1, int nb = -1) {
+        if (!PF) issue(fa, fb);
+        if (fb >= 0) zw_combine<S64, TW, true>(vs, raw, lane, j);
+        else zw_combine<S64, TW, false>(vs, raw, lane, j);
+        if (PF && na >= 0) {
+            zw_after(vs);
+            issue(na, nb);
         }
         // position 0 is slot 0 of lane 0 in both layouts
         if (lane == 0) vs[0] = {vs[0].re * dca, vs[0].im * dcb};
@@ -1819,26 +1828,26 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     // of (a + i b) leaks roundoff of the larger of a, b into the other, and u, w
     // are many orders above g.
     // g = R2C^T gin: C2R with the DC counted once, times 1/2
-    inverse(6, 7, two, two);
+    inverse(6, 7, two, two, 8, -1);
 #pragma unroll
     for (int e = 0; e < NP; ++e) {
         g0[e] = vp[e].re * half;
         g1[e] = vp[e].im * half;
     }
     __syncthreads();
-    inverse(8, -1, two, one);  // (gz, 0)
+    inverse(8, -1, two, one, 4, 5);  // (gz, 0)
 #pragma unroll
     for (int e = 0; e < NP; ++e) r0[e] = vp[e].re * half;  // g2
     __syncthreads();
     // W before U, holding W1, W2 and then U0 rather than partial products
-    inverse(4, 5, one, one);  // (wy, wz), held
+    inverse(4, 5, one, one, 3, 0);  // (wy, wz), held
 #pragma unroll
     for (int e = 0; e < NP; ++e) {
         r1[e] = vp[e].re * s_u;  // W1
         r2[e] = vp[e].im * s_u;  // W2
     }
     __syncthreads();
-    inverse(3, 0, one, one);  // (wx, ux): gU complete; U0 held
+    inverse(3, 0, one, one, 1, 2);  // (wx, ux): gU complete; U0 held
 #pragma unroll
     for (int e = 0; e < NP; ++e) {
         const TW w0 = vp[e].re * s_u, u0 = vp[e].im * s_u, gz = r0[e], w1 = r1[e], w2 = r2[e];
@@ -2249,9 +2258,13 @@ int launch_z_vjp_L(Prec wp, const void *in, void *out, long long lines, int hp, 
         // 18.7 ms); mixed 10.94 -> 10.09 ms with the spectral side in layout
         // 64 (112 VGPRs, four waves; layout 48 10.33 ms).  The twiddle powers
         // formed once per wave keep those occupancies (206 / 119 VGPRs):
-        // reverse step 16.84 -> 16.71 ms FP64, 9.98 -> 9.91 ms mixed.
+        // reverse step 16.84 -> 16.71 ms FP64, 9.98 -> 9.91 ms mixed.  At two
+        // waves the FP64 kernel has room to load each next field pair during
+        // the current one's transforms (248 VGPRs, no spill): 16.91 -> 16.55
+        // ms; in FP32 the same prefetch spills 32 B at four waves and ran
+        // 9.91 -> 10.45 ms.
         if (wp == F64)
-            hipLaunchKernelGGL((k_z_vjp_w<double, false, 2, true>), dim3((unsigned)lines), dim3(64), 0, st,
+            hipLaunchKernelGGL((k_z_vjp_w<double, false, 2, true, true>), dim3((unsigned)lines), dim3(64), 0, st,
                                reinterpret_cast<const cpx<double> *>(in), reinterpret_cast<cpx<double> *>(out), lines,
                                hp, fs_in, fs_out, s_u, s_o, reinterpret_cast<const cpx<double> *>(tw));
         else
