@@ -321,16 +321,74 @@ __device__ __forceinline__ void stnt(cpx<R> *p, cpx<R> v) {
     const V x = {v.re, v.im};
     __builtin_nontemporal_store(x, reinterpret_cast<V *>(p));
 }
-template <int STG, int CH, typename TS, bool XO = false>
-__device__ __forceinline__ void stage_chunk(const StageArgs &sa, bool st, const long long (&idx)[CH],
-                                            const int (&q)[CH], const cpx<TS> (&nn)[CH][3],
-                                            cpx<TS> (*xo)[3] = nullptr, const double *k2 = nullptr) {
-    cpx<TS> *out = reinterpret_cast<cpx<TS> *>(sa.out);
+// x, held at this point in the program: an empty asm that takes and returns
+// the value, so nothing it depends on is moved below it (into a branch)
+template <typename R>
+__device__ __forceinline__ R pinned(R x) {
+    asm volatile("" : "+v"(x));
+    return x;
+}
+// The operands a chunk of CH modes reads from memory: up to four ETD table
+// entries and up to three state arrays (three fields each) per mode; which
+// ones depends on the stage.  stage_load issues the loads, stage_apply forms
+// and stores the results, so a caller can issue the next chunk's loads before
+// this chunk's stores.
+template <int CH, typename TS>
+struct StageOps {
+    double t[4][CH];
+    cpx<TS> s[3][CH][3];
+};
+template <int STG, int CH, typename TS>
+__device__ __forceinline__ void stage_load(const StageArgs &sa, const long long (&idx)[CH], const int (&q)[CH],
+                                           StageOps<CH, TS> &o) {
     const cpx<TS> *u = reinterpret_cast<const cpx<TS> *>(sa.u);
-    cpx<TS> *n1 = reinterpret_cast<cpx<TS> *>(sa.n1);
-    cpx<TS> *sacc = reinterpret_cast<cpx<TS> *>(sa.sacc);
+    const cpx<TS> *n1 = reinterpret_cast<const cpx<TS> *>(sa.n1);
+    const cpx<TS> *sacc = reinterpret_cast<const cpx<TS> *>(sa.sacc);
+    const cpx<TS> *xin = reinterpret_cast<const cpx<TS> *>(sa.xin);
     const double *tab = sa.tab;
     const int ts = sa.tab_stride;
+    const long long fs = sa.fs;
+#pragma unroll
+    for (int i = 0; i < CH; ++i) {
+        if constexpr (STG >= RK_1 && STG <= RK_4) {
+            // s0 the stage input, s1 u, s2 N1 (RK_2) or the accumulator (RK_3, RK_4)
+#pragma unroll
+            for (int f = 0; f < 3; ++f) {
+                o.s[0][i][f] = ldnt(xin + f * fs + idx[i]);
+                if constexpr (STG != RK_1) o.s[1][i][f] = ldnt(u + f * fs + idx[i]);
+                if constexpr (STG == RK_2) o.s[2][i][f] = ldnt(n1 + f * fs + idx[i]);
+                if constexpr (STG == RK_3 || STG == RK_4) o.s[2][i][f] = ldnt(sacc + f * fs + idx[i]);
+            }
+        } else if constexpr (STG == ST_1 || STG == ST_2) {
+            o.t[0][i] = tab[etd_index(ETD_EH, q[i], ts)];
+            o.t[1][i] = tab[etd_index(ETD_A21, q[i], ts)];
+#pragma unroll
+            for (int f = 0; f < 3; ++f) o.s[0][i][f] = ldnt(u + f * fs + idx[i]);
+        } else if constexpr (STG == ST_3) {
+            o.t[0][i] = tab[etd_index(ETD_E, q[i], ts)];
+            o.t[1][i] = tab[etd_index(ETD_A41, q[i], ts)];
+            o.t[2][i] = tab[etd_index(ETD_B1, q[i], ts)];
+            o.t[3][i] = tab[etd_index(ETD_B23, q[i], ts)];
+#pragma unroll
+            for (int f = 0; f < 3; ++f) {
+                o.s[0][i][f] = ldnt(u + f * fs + idx[i]);
+                o.s[1][i][f] = ldnt(sacc + f * fs + idx[i]);
+                o.s[2][i][f] = ldnt(n1 + f * fs + idx[i]);
+            }
+        } else if constexpr (STG == ST_4) {
+            o.t[0][i] = tab[etd_index(ETD_B4, q[i], ts)];
+#pragma unroll
+            for (int f = 0; f < 3; ++f) o.s[0][i][f] = ldnt(sacc + f * fs + idx[i]);
+        }
+    }
+}
+template <int STG, int CH, typename TS, bool XO = false>
+__device__ __forceinline__ void stage_apply(const StageArgs &sa, bool st, const long long (&idx)[CH],
+                                            const cpx<TS> (&nn)[CH][3], const StageOps<CH, TS> &o,
+                                            cpx<TS> (*xo)[3] = nullptr, const double *k2 = nullptr) {
+    cpx<TS> *out = reinterpret_cast<cpx<TS> *>(sa.out);
+    cpx<TS> *n1 = reinterpret_cast<cpx<TS> *>(sa.n1);
+    cpx<TS> *sacc = reinterpret_cast<cpx<TS> *>(sa.sacc);
     const long long fs = sa.fs;
     if constexpr (STG == ST_RAW) {
         if (st) {
@@ -344,18 +402,8 @@ __device__ __forceinline__ void stage_chunk(const StageArgs &sa, bool st, const 
         // nu k^2 state; RK_1: N1 <- r1, x <- u + dt/2 r1; RK_2: acc <- N1 +
         // 2 r2, x <- u + dt/2 r2; RK_3: acc <- acc + 2 r3, x <- u + dt r3;
         // RK_4: u <- u + dt/6 (acc + r4)
-        const cpx<TS> *xin = reinterpret_cast<const cpx<TS> *>(sa.xin);
         const double dt = sa.rk_dt, nu = sa.nu;
-        cpx<TS> stv[CH][3], uu[CH][3], ac[CH][3];
-#pragma unroll
-        for (int i = 0; i < CH; ++i)
-#pragma unroll
-            for (int f = 0; f < 3; ++f) {
-                stv[i][f] = ldnt(xin + f * fs + idx[i]);
-                if constexpr (STG != RK_1) uu[i][f] = ldnt(u + f * fs + idx[i]);
-                if constexpr (STG == RK_2) ac[i][f] = ldnt(n1 + f * fs + idx[i]);
-                if constexpr (STG == RK_3 || STG == RK_4) ac[i][f] = ldnt(sacc + f * fs + idx[i]);
-            }
+        const auto &stv = o.s[0], &uu = o.s[1], &ac = o.s[2];
         if (st) {
 #pragma unroll
             for (int i = 0; i < CH; ++i) {
@@ -384,15 +432,8 @@ __device__ __forceinline__ void stage_chunk(const StageArgs &sa, bool st, const 
         }
     } else if constexpr (STG == ST_1 || STG == ST_2) {
         const double dt = sa.sc->dt;
-        double eh[CH], a[CH];
-        cpx<TS> uu[CH][3];
-#pragma unroll
-        for (int i = 0; i < CH; ++i) {
-            eh[i] = tab[etd_index(ETD_EH, q[i], ts)];
-            a[i] = tab[etd_index(ETD_A21, q[i], ts)];
-#pragma unroll
-            for (int f = 0; f < 3; ++f) uu[i][f] = ldnt(u + f * fs + idx[i]);
-        }
+        const auto &eh = o.t[0], &a = o.t[1];
+        const auto &uu = o.s[0];
         if (st) {
 #pragma unroll
             for (int i = 0; i < CH; ++i)
@@ -415,20 +456,20 @@ __device__ __forceinline__ void stage_chunk(const StageArgs &sa, bool st, const 
         // stage c, and the part of the final combine that does not involve N4:
         // sacc <- e u + dt (b1 N1 + b23 (N2 + N3)), so stage 4 reads one array
         const double dt = sa.sc->dt;
-        double e[CH], a[CH], b1[CH], b23[CH];
-        cpx<TS> uu[CH][3], s2[CH][3], m1[CH][3];
+        double b1[CH], b23[CH];
+        cpx<TS> s2[CH][3];
+        const auto &e = o.t[0], &a = o.t[1];
+        const auto &uu = o.s[0], &m1 = o.s[2];
+        // b1, b23 and N2 enter only the accumulator, whose store is
+        // predicated; held here, their loads go out with the chunk's others
+        // instead of after those have returned (one memory round trip per
+        // chunk, not two)
 #pragma unroll
         for (int i = 0; i < CH; ++i) {
-            e[i] = tab[etd_index(ETD_E, q[i], ts)];
-            a[i] = tab[etd_index(ETD_A41, q[i], ts)];
-            b1[i] = tab[etd_index(ETD_B1, q[i], ts)];
-            b23[i] = tab[etd_index(ETD_B23, q[i], ts)];
+            b1[i] = pinned(o.t[2][i]);
+            b23[i] = pinned(o.t[3][i]);
 #pragma unroll
-            for (int f = 0; f < 3; ++f) {
-                uu[i][f] = ldnt(u + f * fs + idx[i]);
-                s2[i][f] = ldnt(sacc + f * fs + idx[i]);
-                m1[i][f] = ldnt(n1 + f * fs + idx[i]);
-            }
+            for (int f = 0; f < 3; ++f) s2[i][f] = {pinned(o.s[1][i][f].re), pinned(o.s[1][i][f].im)};
         }
         if (st) {
 #pragma unroll
@@ -455,14 +496,8 @@ __device__ __forceinline__ void stage_chunk(const StageArgs &sa, bool st, const 
     } else {  // ST_4: final combine into u, from the stage-3 partial in sacc
         static_assert(STG == ST_4, "unknown stage");
         const double dt = sa.sc->dt;
-        double b4[CH];
-        cpx<TS> p[CH][3];
-#pragma unroll
-        for (int i = 0; i < CH; ++i) {
-            b4[i] = tab[etd_index(ETD_B4, q[i], ts)];
-#pragma unroll
-            for (int f = 0; f < 3; ++f) p[i][f] = ldnt(sacc + f * fs + idx[i]);
-        }
+        const auto &b4 = o.t[0];
+        const auto &p = o.s[0];
 #pragma unroll
         for (int i = 0; i < CH; ++i)
 #pragma unroll
@@ -474,6 +509,15 @@ __device__ __forceinline__ void stage_chunk(const StageArgs &sa, bool st, const 
                 if (st && (!XO || out)) stnt(out + f * fs + idx[i], y);
             }
     }
+}
+// one chunk's loads, then its results
+template <int STG, int CH, typename TS, bool XO = false>
+__device__ __forceinline__ void stage_chunk(const StageArgs &sa, bool st, const long long (&idx)[CH],
+                                            const int (&q)[CH], const cpx<TS> (&nn)[CH][3],
+                                            cpx<TS> (*xo)[3] = nullptr, const double *k2 = nullptr) {
+    StageOps<CH, TS> o;
+    stage_load<STG, CH, TS>(sa, idx, q, o);
+    stage_apply<STG, CH, TS, XO>(sa, st, idx, nn, o, xo, k2);
 }
 
 // Leray projection of one mode (vonkarman-core/src/spectral_ops.rs:114-151)
