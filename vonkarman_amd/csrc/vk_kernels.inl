@@ -322,7 +322,8 @@ __device__ __forceinline__ void stnt(cpx<R> *p, cpx<R> v) {
     __builtin_nontemporal_store(x, reinterpret_cast<V *>(p));
 }
 // x, held at this point in the program: an empty asm that takes and returns
-// the value, so nothing it depends on is moved below it (into a branch)
+// the value, so nothing it depends on is moved below it (into a branch), and
+// nothing computed from it is merged with values computed before it
 template <typename R>
 __device__ __forceinline__ R pinned(R x) {
     asm volatile("" : "+v"(x));
@@ -972,6 +973,8 @@ __global__ __launch_bounds__((FusedGeom<L, STG, TW>::NT)) __attribute__((amdgpu_
     constexpr bool HEAVY = STG == ST_3;
     // kept modes per epilogue chunk.  FP64 takes one: two spilled 40 B per
     // lane in stages 1-2 (fused pass 0.663 -> 0.648 ms with one, same box).
+    // Issuing each next chunk's loads before this chunk's stores (a two-deep
+    // pipeline) spilled 80-392 B per lane, as did larger chunks.
     constexpr int CH = sizeof(TW) == 8 || HEAVY ? 1 : 4;
     static_assert(K % CH == 0, "chunks must tile the kept modes");
     constexpr int TWN = tw_entries<L, PL>();
@@ -1070,10 +1073,15 @@ __global__ __launch_bounds__((FusedGeom<L, STG, TW>::NT)) __attribute__((amdgpu_
         for (int e = 0; e < E; ++e) v[e] = value(e);
         fft_reg_lt<L, +1, T, NT, T, 1, RL>(v, lds, ltw, tid);
         if (PADST || colok) {
+            // the store rows from a fresh copy of the thread index: the
+            // compiler otherwise keeps them from the load phase through the
+            // whole pass (stage 3 spilled 96 B per lane in FP64, 64 B in FP32;
+            // 72 B and 20 B with this, the same time)
+            const int t = pinned(tid);
 #pragma unroll
             for (int e = 0; e < E; ++e)
             {
-                cpx<TW> *dst = aout + fo * gi.out_fs + cinv + (long long)ci_pos_out<L, RL::last, T, NT>(tid, e) * gi.out_ls;
+                cpx<TW> *dst = aout + fo * gi.out_fs + cinv + (long long)ci_pos_out<L, RL::last, T, NT>(t, e) * gi.out_ls;
                 stg_nt(dst, v[e]);
             }
         }
