@@ -972,7 +972,8 @@ __global__ __launch_bounds__((FusedGeom<L, STG, TW>::NT)) __attribute__((amdgpu_
     static_assert(RP % 3 == 0 && 3 * LOUT == 2 * L, "stage pass needs a 3/2-padded plan");
     constexpr bool HEAVY = STG == ST_3;
     // kept modes per epilogue chunk.  FP64 takes one: two spilled 40 B per
-    // lane in stages 1-2 (fused pass 0.663 -> 0.648 ms with one, same box).
+    // lane in stages 1-2 (fused pass 0.663 -> 0.648 ms with one, same box;
+    // with the current epilogue two spill 24 B and ran 0.632 -> 0.644 ms).
     // Issuing each next chunk's loads before this chunk's stores (a two-deep
     // pipeline) spilled 80-392 B per lane, as did larger chunks.
     constexpr int CH = sizeof(TW) == 8 || HEAVY ? 1 : 4;
