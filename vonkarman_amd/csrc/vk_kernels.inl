@@ -79,13 +79,14 @@ struct InvPlan<L, INV_PLAIN, TW> {
     using type = PlanS<L>;
 };
 // INV_U3 (the CFL pass's x and y inverses, and the VJPs' state passes): the
-// low-pass-count plan, except FP64 at power-of-two lengths, where the
-// 8-element plan leaves room to load each next field during a transform (the
-// CFL pass at 256: 152 -> 112 VGPRs, 3 -> 4 waves per SIMD; FP64 step 112.4
-// -> 112.9 steps/s with the CFL pass overlapping stage 1, same boxes)
+// low-pass-count plan in FP32; in FP64 the smaller-E plan, which leaves room
+// to load each next field during a transform (the CFL pass at 256: 152 ->
+// 112 VGPRs, 3 -> 4 waves per SIMD; FP64 step 112.4 -> 112.9 steps/s with the
+// CFL pass overlapping stage 1, same boxes; the VJPs' 384-point instance
+// 248 -> 150 VGPRs, reverse step unchanged)
 template <int L, typename TW>
 struct InvPlan<L, INV_U3, TW> {
-    using type = std::conditional_t<sizeof(TW) == 8 && (L & (L - 1)) == 0, typename PlanSmall<L>::type, PlanS<L>>;
+    using type = std::conditional_t<sizeof(TW) == 8, typename PlanSmall<L>::type, PlanS<L>>;
 };
 // the FP32 y inverse runs the 24-element plan too (4-wave blocks of 16
 // columns; 0.362 -> 0.339 ms at 384), and prefetches like the 12-element one
