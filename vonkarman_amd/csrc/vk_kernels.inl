@@ -1942,6 +1942,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
 // instruction in FP32).  Each block's maximum goes to one of kMaxSlots slots
 // (block index modulo the count, one 128-B line each), and one small block
 // reduces the slots: one atomic per block on a single address serialises.
+// Tiles of 4 or 8 lines (2- and 4-wave blocks) made the CFL pass slower at
+// 256^3 (mixed 0.255 -> 0.285 ms per step, FP64 0.467 -> 0.480).
 template <int L, typename TW>
 using ZMaxGeom = Geom<L, PlanSmall<L>::type::E, TW, 2 * 2 * (int)sizeof(TW)>;
 // The standalone C2R and R2C along the contiguous axis use the z pass's tile:
