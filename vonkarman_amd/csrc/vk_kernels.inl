@@ -1826,13 +1826,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     const TW one = TW(1), two = TW(2), half = TW(0.5);
     cpx<TW> vs[NS], vp[NP];
     TW g0[NP], g1[NP], r0[NP], r1[NP], r2[NP], r3[NP];
-    // inverse pair transform of fields (fa, fb) (fb < 0: none) with per-half
-    // DC factors: the z transpose of an R2C counts its k=0 term once where the
-    // C2R extension would count it twice
-    // The pairs in order: (6, 7), (8, none), (4, 5), (3, 0), (1, 2).  With PF
-    // each next pair's loads are issued once the current pair is combined, so
-    // they are in flight during its transform (and the forward transforms
-    // between the inverse ones).
+    // The input pairs in order: (6, 7), (8, none), (4, 5), (3, 0), (1, 2).
+    // With PF each next pair's loads (na, nb) are issued once the current pair
+    // is combined, so they are in flight during its transform (and the
+    // forward transforms between the inverse ones).
     ZRaw<TW> raw[NS];
     auto issue = [&](int fa, int fb) {
         const gptr<cpx<TW>> pa = opaque(in) + fa * fs_in;
@@ -1840,6 +1837,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
         else zw_issue<S64, TW, false>(raw, pa, pa, base, lane, j);
     };
     if (PF) issue(6, 7);
+    // inverse pair transform of fields (fa, fb) (fb < 0: none) with per-half
+    // DC factors: the z transpose of an R2C counts its k=0 term once where the
+    // C2R extension would count it twice
     auto inverse = [&](int fa, int fb, TW dca, TW dcb, int na = -1, int nb = -1) {
         if (!PF) issue(fa, fb);
         if (fb >= 0) zw_combine<S64, TW, true>(vs, raw, lane, j);
