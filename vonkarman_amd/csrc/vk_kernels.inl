@@ -2275,7 +2275,8 @@ int launch_z_nl_L(Prec wp, const void *in, void *out, long long lines, int hp, l
         //   (92 VGPRs, five waves): 0.326 ms; two lines per block 0.331,
         //   without prefetch 0.347, layout 48 0.356.  The twiddle powers formed
         //   once per wave (99 VGPRs, four waves): 0.326 -> 0.321 ms; a 5-wave
-        //   floor spills 24 B and ran 0.355 ms.
+        //   floor spills 24 B and ran 0.355 ms; two lines per block with
+        //   their c2 paired (105 VGPRs, four waves) 0.327 ms.
         if (wp == F64) {
             if (lines % 2 == 0)
                 hipLaunchKernelGGL((k_z_nl_w<double, false, false, 3, 2>), dim3((unsigned)(lines / 2)), dim3(64), 0,
