@@ -2270,7 +2270,8 @@ int launch_z_nl_L(Prec wp, const void *in, void *out, long long lines, int hp, l
         //   (c2 paired; 166 VGPRs).  Layout 64 spills 44 B at that floor
         //   (0.742 ms), prefetch at two waves ran 0.63-0.70, and issuing the
         //   second line's loads during the first line's forward transforms
-        //   measured the same.
+        //   measured the same.  Cached twiddle powers (as FP32) spill 40 B
+        //   at this floor.
         //   FP32: spectral side in layout 64 with prefetch, one line per block
         //   (92 VGPRs, five waves): 0.326 ms; two lines per block 0.331,
         //   without prefetch 0.347, layout 48 0.356.  The twiddle powers formed
