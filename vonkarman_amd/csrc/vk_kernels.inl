@@ -338,8 +338,9 @@ __device__ __forceinline__ R pinned(R x) {
 // The operands a chunk of CH modes reads from memory: up to four ETD table
 // entries and up to three state arrays (three fields each) per mode; which
 // ones depends on the stage.  stage_load issues the loads, stage_apply forms
-// and stores the results, so a caller can issue the next chunk's loads before
-// this chunk's stores.
+// and stores the results; stage_chunk runs the two in turn.  (Issuing each
+// next chunk's loads before this chunk's stores spilled the fused pass at two
+// waves per SIMD in every stage.)
 template <int CH, typename TS>
 struct StageOps {
     double t[4][CH];
